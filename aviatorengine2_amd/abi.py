@@ -1,0 +1,175 @@
+"""Byte layouts of the C ABI (include/avrt.h) as numpy dtypes, and the ctypes
+binding of libavrt.so.
+
+The dtypes restate the reference's GPU structs field for field:
+Object_GPU (scene/gpu_types.h:26-81), BVHNode_GPU (scene/bvh.h:31-41),
+MeshInstance_GPU (scene/gpu_types.h:84-95, with `scale` in pad0),
+Vertex_GPU (scene/mesh_library.h:39-44), AABB (scene/aabb.h:14-18) and the
+Config cbuffer (renderer/renderer.h:76-148).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_F = "<f4"
+_U = "<u4"
+_I = "<i4"
+
+OBJECT_DTYPE = np.dtype([
+    ("x", _F), ("y", _F), ("z", _F), ("prevRotX", _F),
+    ("x2", _F), ("y2", _F), ("z2", _F), ("radius", _F),
+    ("u_x", _F), ("u_y", _F), ("u_z", _F), ("prevRotY", _F),
+    ("v_x", _F), ("v_y", _F), ("v_z", _F), ("prevRotZ", _F),
+    ("half_x", _F), ("half_y", _F), ("half_z", _F), ("rotation", _F),
+    ("density", _F), ("emission", _F), ("instanceIndex", _U), ("textureTint", _F),
+    ("r", _F), ("g", _F), ("b", _F), ("fuzz", _F),
+    ("refraction", _F), ("shapeType", _U), ("colorType", _U), ("textureID", _U),
+])
+assert OBJECT_DTYPE.itemsize == 128
+
+NODE_DTYPE = np.dtype([
+    ("min_x", _F), ("min_y", _F), ("min_z", _F), ("pad0", _F),
+    ("max_x", _F), ("max_y", _F), ("max_z", _F),
+    ("left", _I), ("right", _I), ("count", _I), ("padding", _I), ("pad_trailing", _I),
+])
+assert NODE_DTYPE.itemsize == 48
+
+INSTANCE_DTYPE = np.dtype([
+    ("px", _F), ("py", _F), ("pz", _F), ("scale", _F),
+    ("qx", _F), ("qy", _F), ("qz", _F), ("qw", _F),
+    ("vertexBase", _U), ("indexBase", _U), ("triangleCount", _U), ("blasBase", _U),
+])
+assert INSTANCE_DTYPE.itemsize == 48
+
+VERTEX_DTYPE = np.dtype([
+    ("px", _F), ("py", _F), ("pz", _F), ("nx", _F), ("ny", _F), ("nz", _F), ("u", _F), ("v", _F),
+])
+assert VERTEX_DTYPE.itemsize == 32
+
+AABB_DTYPE = np.dtype([
+    ("min_x", _F), ("min_y", _F), ("min_z", _F), ("max_x", _F), ("max_y", _F), ("max_z", _F),
+])
+assert AABB_DTYPE.itemsize == 24
+
+CONFIG_DTYPE = np.dtype([
+    ("source_x", _F), ("source_y", _F), ("source_z", _F), ("fov", _F),
+    ("target_x", _F), ("target_y", _F), ("target_z", _F), ("focus_dist", _F),
+    ("defocus_angle", _F), ("padding", _F, (3,)),
+    ("up_x", _F), ("up_y", _F), ("up_z", _F), ("oof", _F),
+    ("sky_r", _F), ("sky_g", _F), ("sky_b", _F), ("width", _U),
+    ("horizon_r", _F), ("horizon_g", _F), ("horizon_b", _F), ("height", _U),
+    ("samples", _U), ("batches", _U), ("batch", _U), ("depth", _U),
+    ("num_spheres", _U), ("renderType", _U), ("num_lights", _U), ("exposure", _F),
+    ("debugView", _U), ("zNear", _F), ("zFar", _F), ("jitter_x", _F),
+    ("jitter_y", _F), ("displayOnly", _U), ("traceScale", _F), ("frameIndex", _U),
+    ("historyReset", _U), ("cameraMoved", _U), ("historyCap", _F), ("padding3", _F),
+    ("prevViewProj", _F, (16,)),
+])
+assert CONFIG_DTYPE.itemsize == 240
+
+# Enum values (scene/material.h:11-18, scene/gpu_types.h:14-21).
+LAMBERTIAN, METAL, DIELECTRIC, DIFFUSE_LIGHT, ISOTROPIC = 0, 1, 2, 3, 4
+SPHERE, QUAD, BOX, TRIANGLE, MESH = 0, 1, 2, 3, 4
+NO_TEXTURE = 0xFFFFFFFF
+MODE_PATH_TRACE = 0
+MODE_RAY_TRACE = 1
+
+AVRT_OK = 0
+ERRORS = {
+    -1: "AVRT_E_INVALID_ARG", -2: "AVRT_E_CAPACITY", -3: "AVRT_E_OOM", -4: "AVRT_E_DEVICE",
+    -5: "AVRT_E_STATE", -6: "AVRT_E_BUDGET", -7: "AVRT_E_UNSUPPORTED",
+}
+
+
+class AvrtCreateOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int), ("max_width", ctypes.c_uint32), ("max_height", ctypes.c_uint32),
+        ("max_paths", ctypes.c_uint32), ("fp16_accum", ctypes.c_uint32),
+        ("traversal_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
+    ]
+
+
+class AvrtStats(ctypes.Structure):
+    _fields_ = [
+        ("rays_primary", ctypes.c_uint64), ("rays_bounce", ctypes.c_uint64),
+        ("rays_shadow", ctypes.c_uint64), ("paths", ctypes.c_uint64),
+        ("ms_total", ctypes.c_float), ("ms_extend", ctypes.c_float), ("ms_shade", ctypes.c_float),
+        ("ms_shadow", ctypes.c_float), ("ms_other", ctypes.c_float),
+        ("launches_extend", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+# Every entry point include/avrt.h declares: name -> (restype, argtypes).
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SIG = {
+    "avrt_create": (ctypes.c_int, [ctypes.POINTER(AvrtCreateOptions), ctypes.POINTER(_P)]),
+    "avrt_destroy": (ctypes.c_int, [_P]),
+    "avrt_last_error": (ctypes.c_char_p, [_P]),
+    "avrt_upload_geometry": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _P, _U32]),
+    "avrt_upload_scene": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _U32]),
+    "avrt_upload_textures": (ctypes.c_int, [_P, _P, _U32, _U32, _U32]),
+    "avrt_render": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+    "avrt_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(AvrtStats)]),
+    "avrt_read_image": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "avrt_read_display": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "avrt_trace_primary": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "avrt_build_bvh": (ctypes.c_int, [_P, _U32, _P, ctypes.POINTER(_U32), _P]),
+    "avrt_build_blas": (ctypes.c_int, [_P, _U32, _P, _U32, _P, ctypes.POINTER(_U32), _P, _P]),
+    "avrt_instance_bounds": (ctypes.c_int, [_P, _U32, _P, _P]),
+    "avrt_scene_set_authoring": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _P]),
+    "avrt_scene_step": (ctypes.c_int, [_P, _P, ctypes.c_float]),
+    "avrt_scene_read": (ctypes.c_int, [_P, _P, ctypes.POINTER(_U32), _P, _P, _P]),
+    "avrt_refit_blas": (ctypes.c_int, [_P]),
+    "avrt_update_vertices": (ctypes.c_int, [_P, _P, _U32, _U32]),
+    "avrt_read_blas": (ctypes.c_int, [_P, _P, _U32]),
+    "avrt_version": (ctypes.c_char_p, []),
+}
+
+PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PACKAGE_DIR, "libavrt.so")
+
+_lib = None
+
+
+class AvrtError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load libavrt.so (built in-tree by aviatorengine2_amd.build). Raises if
+    it is missing: there is no fallback path."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing; build it with aviatorengine2_amd.build.build_library()")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIG.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIG)
+
+
+def ptr(a: np.ndarray | None):
+    """ctypes pointer of a C-contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)

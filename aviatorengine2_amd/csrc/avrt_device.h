@@ -1,0 +1,444 @@
+// avrt_device.h - device functions of the wavefront path tracer (gfx950).
+//
+// Everything here restates the per-ray arithmetic of the reference shaders
+// with the same float operations in the same order, so results are
+// bit-identical to a strict-FP32 evaluation of the HLSL (the library is built
+// with -ffp-contract=off, correctly rounded division and square root):
+//   Random / RandomDirection          ray_trace.comp.hybrid.hlsl:238-251
+//   IntersectAABB                     :547-562
+//   SphereSpan / HitSphere            :285-298, 382-423
+//   HitQuad / HitTriangle             :425-513
+//   BoxIntersect / HitBox             :315-348, 729-773
+//   quat_rotate / ToLocal             :518-543
+//   HitMeshTriangle / HitMesh         :564-707
+//   TraverseBVH                       :862-917
+// Where HLSL leaves the result open the definitions are: normalize(v) =
+// v * (1/sqrt(dot(v,v))); sin/cos by Cody-Waite reduction and Taylor
+// polynomials (rt_sincos); IEEE minNum/maxNum for min/max.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "avrt_internal.h"
+
+namespace avrt {
+namespace dev {
+
+struct F3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ F3 operator*(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ F3 operator/(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b)
+{
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(F3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ F3 normalize(F3 a)
+{
+    const float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ F3 lerp(F3 a, F3 b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ F3 vmin(F3 a, F3 b) { return f3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+__device__ __forceinline__ F3 vmax(F3 a, F3 b) { return f3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+__device__ __forceinline__ float saturate(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ F3 reflect(F3 i, F3 n) { return i - 2.0f * n * dot(i, n); }
+__device__ __forceinline__ F3 refract(F3 i, F3 n, float eta)
+{
+    const float d = dot(n, i);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return f3(0.0f, 0.0f, 0.0f);
+    return eta * i - (eta * d + sqrtf(k)) * n;
+}
+__device__ __forceinline__ float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ F3 ld3(const float *p) { return f3(p[0], p[1], p[2]); }
+
+// sin and cos of a: reduction a = k*pi/2 + x with pi/2 split in three parts
+// (Cody-Waite), then Taylor polynomials of sin (degree 9) and cos (degree 10)
+// on |x| <= pi/4, quadrant selected from k.
+__device__ __forceinline__ void rt_sincos(float a, float *s, float *c)
+{
+    const float k = rintf(a * 0.636619772f);
+    float x = a - k * 1.5703125f;
+    x = x - k * 4.837512969970703125e-4f;
+    x = x - k * 7.54978995489188216e-8f;
+    const float x2 = x * x;
+    float ps = 2.75573192e-6f;
+    ps = ps * x2 - 1.98412698e-4f;
+    ps = ps * x2 + 8.33333333e-3f;
+    ps = ps * x2 - 1.66666667e-1f;
+    const float sn = x + (x * x2) * ps;
+    float pc = -2.75573192e-7f;
+    pc = pc * x2 + 2.48015873e-5f;
+    pc = pc * x2 - 1.38888889e-3f;
+    pc = pc * x2 + 4.16666667e-2f;
+    pc = pc * x2 - 0.5f;
+    const float cs = 1.0f + x2 * pc;
+    const int q = static_cast<int>(k) & 3;
+    const float ss = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
+    const float cc = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
+    *s = ss;
+    *c = cc;
+}
+
+// PCG-hash stream (ray_trace.comp.hybrid.hlsl:238-243).
+struct Rng {
+    uint32_t s;
+    __device__ __forceinline__ float next()
+    {
+        s = s * 747796405u + 2891336453u;
+        const uint32_t x = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+        return static_cast<float>((x >> 22u) ^ x) / 4294967296.0f;
+    }
+    __device__ __forceinline__ F3 direction()
+    {
+        const float z = next() * 2.0f - 1.0f;
+        const float a = next() * 6.2831853f;
+        const float r = sqrtf(1.0f - z * z);
+        float sn, cs;
+        rt_sincos(a, &sn, &cs);
+        return f3(r * cs, r * sn, z);
+    }
+};
+
+__device__ __forceinline__ F3 quat_rotate(float qx, float qy, float qz, float qw, F3 v)
+{
+    const F3 u = f3(qx, qy, qz);
+    const F3 t = 2.0f * cross(u, v);
+    return v + qw * t + cross(u, t);
+}
+
+__device__ __forceinline__ F3 rotate_y(F3 v, float s, float c) { return f3(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
+
+// ---------------------------------------------------------------------------
+// Traversal stack: a ring of K entries per lane in LDS holding the top of the
+// stack, older entries spilled to a per-lane scratch array. The LDS ring is
+// laid out [slot][lane] so a wave's accesses hit 64 consecutive dwords.
+// ---------------------------------------------------------------------------
+template <int K, int BLOCK, int SPILL>
+struct ShortStack {
+    uint32_t *col; // this lane's column: lds + threadIdx.x
+    uint32_t spill[SPILL > 0 ? SPILL : 1];
+    int sp;
+    int lo; // lowest entry still held in LDS
+    bool overflow;
+
+    __device__ __forceinline__ void init(uint32_t *lds_base)
+    {
+        col = lds_base + threadIdx.x;
+        sp = 0;
+        lo = 0;
+        overflow = false;
+    }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ void push(uint32_t v)
+    {
+        if (K == 0) {
+            if (sp < SPILL) spill[sp++] = v;
+            else overflow = true;
+            return;
+        }
+        if (sp - lo == K) {
+            if (lo >= SPILL) {
+                overflow = true;
+                return;
+            }
+            spill[lo] = col[(lo & (K - 1)) * BLOCK];
+            lo++;
+        }
+        col[(sp & (K - 1)) * BLOCK] = v;
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        sp--;
+        if (K == 0) return spill[sp];
+        if (sp < lo) {
+            lo = sp;
+            return spill[sp];
+        }
+        return col[(sp & (K - 1)) * BLOCK];
+    }
+};
+
+// Slab test of one box (IntersectAABB): t0=(min-o)*inv, t1=(max-o)*inv,
+// tmin = max(max(ts.x,ts.y), max(ts.z,near)), tmax likewise with far.
+__device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, F3 o, F3 inv,
+                                     float near, float far, float *tentry)
+{
+    const float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+    const float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+    const float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+    const float tsx = fminf(t0x, t1x), tbx = fmaxf(t0x, t1x);
+    const float tsy = fminf(t0y, t1y), tby = fmaxf(t0y, t1y);
+    const float tsz = fminf(t0z, t1z), tbz = fmaxf(t0z, t1z);
+    const float tmin = fmaxf(fmaxf(tsx, tsy), fmaxf(tsz, near));
+    const float tmax = fminf(fminf(tbx, tby), fminf(tbz, far));
+    *tentry = tmin;
+    return tmin <= tmax;
+}
+
+struct HitRec {
+    float t;
+    uint32_t obj;   // TLAS slot, ~0u on a miss
+    uint32_t prim;  // BLAS triangle slot for meshes, ~0u otherwise
+    float alpha, beta;
+};
+
+struct SceneDev {
+    const Node2 *tlas;       // super-root at 0 (null when the scene is empty)
+    const ObjGeom *geom;     // per TLAS slot
+    const Node2 *blas;       // all BLAS trees, each with its super-root
+    const TriPre *tris;      // per mesh triangle, index base/3 + slot
+    const AvrtObject *objects;
+    const AvrtVertex *verts;
+    const uint32_t *indices;
+    const uint32_t *lights;
+    const uchar4 *tex;
+    uint32_t nobj;
+};
+
+// Analytic leaf tests from ObjGeom; return true and set *t (and uv for the
+// planar shapes) when the hit is inside (near, far) by each shape's own rule.
+__device__ __forceinline__ bool test_sphere(const ObjGeom &g, F3 o, F3 d, float time, float near, float far, float *t)
+{
+    const F3 centre = f3(g.a[0] + time * g.b[0], g.a[1] + time * g.b[1], g.a[2] + time * g.b[2]);
+    const F3 off = centre - o;
+    const float a = dot(d, d);
+    const float b = dot(d, off);
+    const float c = dot(off, off) - g.a[3] * g.a[3];
+    const float disc = b * b - a * c;
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    float root = (b - sq) / a;
+    if (root <= near || far <= root) {
+        root = (b + sq) / a;
+        if (root <= near || far <= root) return false;
+    }
+    *t = root;
+    return true;
+}
+
+__device__ __forceinline__ bool test_planar(const ObjGeom &g, bool tri, F3 o, F3 d, float near, float far, float *t,
+                                            float *alpha, float *beta)
+{
+    const F3 u = f3(g.b[0], g.b[1], g.b[2]);
+    const F3 v = f3(g.c[0], g.c[1], g.c[2]);
+    const F3 n = cross(u, v);
+    const float denom = dot(n, d);
+    if (fabsf(denom) < 1e-8f) return false;
+    const float tt = (g.a[3] - dot(n, o)) / denom;
+    if (tt < near || tt > far) return false;
+    const F3 p = o + d * tt;
+    const F3 planar = p - f3(g.a[0], g.a[1], g.a[2]);
+    const F3 w = f3(__uint_as_float(g.d[0]), __uint_as_float(g.d[1]), __uint_as_float(g.d[2]));
+    const float al = dot(w, cross(planar, v));
+    const float be = dot(w, cross(u, planar));
+    if (tri) {
+        if (al < 0.0f || be < 0.0f || al + be > 1.0f) return false;
+    } else {
+        if (al < 0.0f || al > 1.0f || be < 0.0f || be > 1.0f) return false;
+    }
+    *t = tt;
+    *alpha = al;
+    *beta = be;
+    return true;
+}
+
+struct BoxLocal {
+    F3 lo, ld;
+    float tin, tout;
+    F3 en, ex;
+};
+
+__device__ __forceinline__ bool box_local(const ObjGeom &g, F3 o, F3 d, BoxLocal &r)
+{
+    const float ys = g.a[3], yc = g.b[3];
+    r.lo = rotate_y(o - f3(g.a[0], g.a[1], g.a[2]), -ys, yc);
+    r.ld = rotate_y(d, -ys, yc);
+    const F3 half = f3(g.b[0], g.b[1], g.b[2]);
+    const F3 inv = f3(1.0f / r.ld.x, 1.0f / r.ld.y, 1.0f / r.ld.z);
+    const F3 t0 = (-half - r.lo) * inv;
+    const F3 t1 = (half - r.lo) * inv;
+    const F3 ts = vmin(t0, t1), tb = vmax(t0, t1);
+    r.tin = fmaxf(fmaxf(ts.x, ts.y), ts.z);
+    r.tout = fminf(fminf(tb.x, tb.y), tb.z);
+    r.en = f3(0.0f, 0.0f, -sgn(r.ld.z));
+    if (r.tin == ts.x) r.en = f3(-sgn(r.ld.x), 0.0f, 0.0f);
+    else if (r.tin == ts.y) r.en = f3(0.0f, -sgn(r.ld.y), 0.0f);
+    r.ex = f3(0.0f, 0.0f, sgn(r.ld.z));
+    if (r.tout == tb.x) r.ex = f3(sgn(r.ld.x), 0.0f, 0.0f);
+    else if (r.tout == tb.y) r.ex = f3(0.0f, sgn(r.ld.y), 0.0f);
+    return r.tout > r.tin;
+}
+
+__device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float near, float far, float *t)
+{
+    BoxLocal r;
+    if (!box_local(g, o, d, r)) return false;
+    float tt = r.tin;
+    if (tt <= near || far <= tt) {
+        tt = r.tout;
+        if (tt <= near || far <= tt) return false;
+    }
+    *t = tt;
+    return true;
+}
+
+// Closest hit through TLAS and BLAS. REF_ORDER=true visits children like the
+// reference (right child first), so equal-t ties resolve exactly as there;
+// false visits the nearer child first.
+template <bool REF_ORDER, int K, int BLOCK, int SPILL>
+__device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, float time, float near, float far,
+                                              uint32_t *lds_stack, HitRec &hit, uint32_t *node_visits)
+{
+    hit.obj = ~0u;
+    hit.prim = ~0u;
+    hit.t = far;
+    if (sc.tlas == nullptr) return false;
+    const F3 winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    F3 ao = o, ad = d, ainv = winv;
+    ShortStack<K, BLOCK, SPILL> st;
+    st.init(lds_stack);
+    bool in_blas = false;
+    uint32_t cur_obj = 0, tri_base = 0;
+    bool found = false;
+    uint32_t ref = 0; // TLAS super-root
+    uint32_t visits = 0;
+    while (true) {
+        if (ref == kRefReturn) {
+            in_blas = false;
+            ao = o;
+            ad = d;
+            ainv = winv;
+        } else if ((ref & kLeafFlag) == 0) {
+            const Node2 *np = (in_blas ? sc.blas : sc.tlas) + ref;
+            const float4 X = *reinterpret_cast<const float4 *>(np->bx);
+            const float4 Y = *reinterpret_cast<const float4 *>(np->by);
+            const float4 Z = *reinterpret_cast<const float4 *>(np->bz);
+            const uint4 R = *reinterpret_cast<const uint4 *>(np->ref);
+            visits++;
+            float e0, e1;
+            const bool h0 = slab(X.x, X.z, Y.x, Y.z, Z.x, Z.z, ao, ainv, near, far, &e0) && R.x != kRefInvalid;
+            const bool h1 = slab(X.y, X.w, Y.y, Y.w, Z.y, Z.w, ao, ainv, near, far, &e1) && R.y != kRefInvalid;
+            if (h0 && h1) {
+                const bool first1 = REF_ORDER ? true : (e1 < e0);
+                st.push(first1 ? R.x : R.y);
+                ref = first1 ? R.y : R.x;
+                continue;
+            }
+            if (h0) {
+                ref = R.x;
+                continue;
+            }
+            if (h1) {
+                ref = R.y;
+                continue;
+            }
+        } else {
+            const uint32_t first = ref & 0x00FFFFFFu;
+            const uint32_t count = (ref >> 24) & 0x7Fu;
+            if (in_blas) {
+                for (uint32_t i = 0; i < count; i++) {
+                    const TriPre *tp = sc.tris + tri_base + first + i;
+                    const float4 A = *reinterpret_cast<const float4 *>(tp->p0);
+                    const float4 B = *reinterpret_cast<const float4 *>(tp->e1);
+                    const float4 C = *reinterpret_cast<const float4 *>(tp->e2);
+                    const float4 W = *reinterpret_cast<const float4 *>(tp->w);
+                    if (__float_as_uint(B.w) == 0u) continue;
+                    const F3 e1 = f3(B.x, B.y, B.z), e2 = f3(C.x, C.y, C.z);
+                    const F3 n = cross(e1, e2);
+                    const float denom = dot(n, ad);
+                    if (fabsf(denom) < 1e-12f) continue;
+                    const float t = (A.w - dot(n, ao)) / denom;
+                    if (t < near || t > far) continue;
+                    const F3 planar = ao + ad * t - f3(A.x, A.y, A.z);
+                    const F3 w = f3(W.x, W.y, W.z);
+                    const float al = dot(w, cross(planar, e2));
+                    const float be = dot(w, cross(e1, planar));
+                    if (al < 0.0f || be < 0.0f || al + be > 1.0f) continue;
+                    far = t;
+                    found = true;
+                    hit.t = t;
+                    hit.obj = cur_obj;
+                    hit.prim = first + i;
+                    hit.alpha = al;
+                    hit.beta = be;
+                }
+            } else {
+                bool descend = false;
+                for (uint32_t i = 0; i < count; i++) {
+                    const uint32_t slot = first + i;
+                    const ObjGeom *gp = sc.geom + slot;
+                    ObjGeom g;
+                    *reinterpret_cast<float4 *>(g.a) = *reinterpret_cast<const float4 *>(gp->a);
+                    *reinterpret_cast<float4 *>(g.b) = *reinterpret_cast<const float4 *>(gp->b);
+                    *reinterpret_cast<float4 *>(g.c) = *reinterpret_cast<const float4 *>(gp->c);
+                    *reinterpret_cast<uint4 *>(g.d) = *reinterpret_cast<const uint4 *>(gp->d);
+                    const uint32_t flags = g.d[3];
+                    if ((flags & kGeomValid) == 0) continue;
+                    const uint32_t shape = flags & 0xFFu;
+                    float t = 0.0f, al = 0.0f, be = 0.0f;
+                    bool h = false;
+                    if (shape == AVRT_SHAPE_MESH) {
+                        // Leave the rest of this leaf for later and walk the
+                        // instance's BLAS in mesh-local space (ToLocal).
+                        if (i + 1 < count) st.push(make_leaf(slot + 1, count - i - 1));
+                        st.push(kRefReturn);
+                        const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];
+                        ao = quat_rotate(qx, qy, qz, qw, o - f3(g.a[0], g.a[1], g.a[2]));
+                        ad = quat_rotate(qx, qy, qz, qw, d);
+                        const float s = g.a[3];
+                        if (s != 1.0f) {
+                            ao = ao / s;
+                            ad = ad / s;
+                        }
+                        ainv = f3(1.0f / ad.x, 1.0f / ad.y, 1.0f / ad.z);
+                        cur_obj = slot;
+                        tri_base = __float_as_uint(g.c[1]);
+                        in_blas = true;
+                        ref = __float_as_uint(g.c[0]);
+                        descend = true;
+                        break;
+                    } else if (shape == AVRT_SHAPE_SPHERE) {
+                        h = test_sphere(g, o, d, time, near, far, &t);
+                    } else if (shape == AVRT_SHAPE_TRIANGLE) {
+                        h = test_planar(g, true, o, d, near, far, &t, &al, &be);
+                    } else if (shape == AVRT_SHAPE_QUAD) {
+                        h = test_planar(g, false, o, d, near, far, &t, &al, &be);
+                    } else {
+                        h = test_box(g, o, d, near, far, &t);
+                    }
+                    if (h) {
+                        far = t;
+                        found = true;
+                        hit.t = t;
+                        hit.obj = slot;
+                        hit.prim = ~0u;
+                        hit.alpha = al;
+                        hit.beta = be;
+                    }
+                }
+                if (descend) continue;
+            }
+        }
+        if (st.empty()) break;
+        ref = st.pop();
+    }
+    if (node_visits) *node_visits = visits;
+    return found;
+}
+
+} // namespace dev
+} // namespace avrt

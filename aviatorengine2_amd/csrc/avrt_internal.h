@@ -1,0 +1,94 @@
+// avrt_internal.h - device-side layouts derived from the reference ABI.
+//
+// The reference ABI arrays (Object_GPU, BVHNode_GPU, Vertex_GPU, meshIndex,
+// MeshInstance_GPU) are accepted unchanged at the C boundary and converted on
+// upload into layouts that suit CDNA4 traversal:
+//
+//   Node2   64 B  one interior node of a binary BVH holding BOTH children's
+//                 boxes, so one node fetch (four dwordx4 loads, half a 128 B
+//                 line) tests two boxes. Child references are tagged:
+//                 interior = node index; leaf = LEAF | count << 24 | first.
+//                 Each tree starts with a "super-root" whose child 0 is the
+//                 reference root (so the root box is tested, bvh.h:275) and
+//                 whose child 1 is empty.
+//   TriPre  64 B  per mesh triangle in BLAS slot order: p0, e1, e2 and
+//                 w = n / |n|^2 plus dot(n, p0), exactly the quantities
+//                 HitMeshTriangle (ray_trace.comp.hybrid.hlsl:564-598)
+//                 derives from the three vertices, computed once with the
+//                 same float operations.
+//   ObjGeom 64 B  per TLAS slot: just what the leaf test of that shape needs
+//                 (HitSphere / HitQuad / HitTriangle / HitBox / HitMesh
+//                 instance transform), so traversal never reads the 128 B
+//                 material record.
+#pragma once
+
+#include <cstdint>
+
+namespace avrt {
+
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr uint32_t kRefInvalid = 0xFFFFFFFFu;   // empty child
+constexpr uint32_t kRefReturn = 0xFFFFFFFEu;    // stack marker: leave a BLAS
+constexpr uint32_t kMaxLeafCount = 127u;
+constexpr uint32_t kMaxLeafFirst = (1u << 24) - 3u;
+
+inline constexpr uint32_t make_leaf(uint32_t first, uint32_t count)
+{
+    return kLeafFlag | (count << 24) | first;
+}
+// A TLAS "instance entry" on the traversal stack: a leaf with count 0.
+inline constexpr uint32_t make_inst_entry(uint32_t slot) { return kLeafFlag | slot; }
+
+struct alignas(16) Node2 {
+    // x: (c0.min, c1.min, c0.max, c1.max), same for y and z.
+    float bx[4];
+    float by[4];
+    float bz[4];
+    uint32_t ref[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(Node2) == 64, "Node2 is 64 bytes");
+
+struct alignas(16) TriPre {
+    float p0[3];
+    float d0;      // dot(n, p0)
+    float e1[3];
+    uint32_t valid; // 0 when |n|^2 < 1e-12 (never hit)
+    float e2[3];
+    float pad0;
+    float w[3];
+    float pad1;
+};
+static_assert(sizeof(TriPre) == 64, "TriPre is 64 bytes");
+
+// ObjGeom word 15 (flags): bits 0-7 shape, bit 8 valid.
+constexpr uint32_t kGeomValid = 1u << 8;
+
+struct alignas(16) ObjGeom {
+    float a[4];
+    float b[4];
+    float c[4];
+    uint32_t d[4];
+};
+static_assert(sizeof(ObjGeom) == 64, "ObjGeom is 64 bytes");
+// Sphere:   a = (P.xyz, radius)  b = (P2 - P, 0)
+// Quad/Tri: a = (P.xyz, dot(n,P)) b = (u.xyz, 0) c = (v.xyz, 0) d.xyz = w as bits
+// Box:      a = (P.xyz, sin yaw)  b = (half.xyz, cos yaw)
+// Mesh:     a = (pos.xyz, scale)  b = (qx,qy,qz,qw)  c = (blas super-root,
+//           triangle base, vertex base, index base) as bits
+
+// Per-frame constants shared by every kernel of a render.
+struct FrameParams {
+    float pixel0[3], pixelU[3], pixelV[3], du[3], dv[3], source[3];
+    float sky[3], horizon[3];
+    float defocus;
+    float exposure;
+    uint32_t width, height, spp, batch, depth;
+    uint32_t num_lights, quad_lights;
+    uint32_t jumpA, jumpC; // RNG jump over the NEE draws (2 per quad light)
+    uint32_t fp16;
+    uint32_t has_texture;
+    uint32_t tex_w, tex_h, tex_layers;
+};
+
+} // namespace avrt

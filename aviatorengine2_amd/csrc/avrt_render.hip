@@ -1,0 +1,1439 @@
+// avrt_render.hip - wavefront path tracer for gfx950 and the C ABI around it.
+//
+// One render = raygen -> [extend -> shade -> shadow] x depth -> resolve.
+//   raygen   one thread per (pixel, sample) path: camera ray and RNG seed of
+//            reference frame Batch = batch + sample (ray_trace.comp.hybrid.hlsl
+//            :1346-1373, path_trace.comp.hlsl:789-811).
+//   extend   closest hit, TLAS -> BLAS, short stack in LDS spilling to scratch
+//            (TraverseBVH / HitMesh, :862-917, :600-707).
+//   shade    hit attributes, emission / sky, next-event setup, BSDF bounce,
+//            Russian roulette (ColorRay :1230-1325; path_trace main :826-849).
+//            Surviving paths are compacted into the next queue with a wave
+//            ballot, a lane prefix count (mbcnt) and one atomic per wave.
+//   shadow   NEE: light sampling replayed from the saved RNG state, shadow rays
+//            stepped through glass (CalculateDirectLight :1122-1212), clamped
+//            contribution added to the path radiance.
+//   resolve  per pixel, the sample frames folded into the accumulation image
+//            in frame order, then ACES + sRGB display (:1380-1403;
+//            path_trace.comp.hlsl:851-859).
+// Ray state is SoA (three float4 streams, double buffered by queue position)
+// so every stage reads and writes it fully coalesced.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/avrt.h"
+#include "avrt_device.h"
+#include "avrt_internal.h"
+
+using namespace avrt;
+using namespace avrt::dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStackLds = 16;
+constexpr int kStackSpill = 96;
+
+__constant__ float c_srgb[256];
+
+// ---------------------------------------------------------------------------
+// Wave-level queue compaction: returns this lane's output slot (valid only if
+// pred) with one atomic add per wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter)
+{
+    const uint64_t active = __ballot(1);
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane_off =
+        __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    const int leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(__lane_id()) == leader && mask != 0ull) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + lane_off;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Texture: bilinear, REPEAT, level 0, per-texel sRGB decode.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+__device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t layer, float u, float v)
+{
+    if (fp.has_texture == 0) return f3(1.0f, 1.0f, 1.0f);
+    if (layer >= fp.tex_layers) layer = fp.tex_layers - 1;
+    if (!isfinite(u)) u = 0.0f;
+    if (!isfinite(v)) v = 0.0f;
+    const float x = u * static_cast<float>(fp.tex_w) - 0.5f;
+    const float y = v * static_cast<float>(fp.tex_h) - 0.5f;
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int W = static_cast<int>(fp.tex_w), H = static_cast<int>(fp.tex_h);
+    const int i0 = wrapi(static_cast<int>(fmodf(x0, static_cast<float>(W))), W);
+    const int j0 = wrapi(static_cast<int>(fmodf(y0, static_cast<float>(H))), H);
+    const int i1 = (i0 + 1) % W, j1 = (j0 + 1) % H;
+    const uchar4 *base = sc.tex + static_cast<size_t>(layer) * W * H;
+    const uchar4 a = base[static_cast<size_t>(j0) * W + i0];
+    const uchar4 b = base[static_cast<size_t>(j0) * W + i1];
+    const uchar4 c = base[static_cast<size_t>(j1) * W + i0];
+    const uchar4 e = base[static_cast<size_t>(j1) * W + i1];
+    const F3 t00 = f3(c_srgb[a.x], c_srgb[a.y], c_srgb[a.z]);
+    const F3 t10 = f3(c_srgb[b.x], c_srgb[b.y], c_srgb[b.z]);
+    const F3 t01 = f3(c_srgb[c.x], c_srgb[c.y], c_srgb[c.z]);
+    const F3 t11 = f3(c_srgb[e.x], c_srgb[e.y], c_srgb[e.z]);
+    const F3 top = t00 + fx * (t10 - t00);
+    const F3 bot = t01 + fx * (t11 - t01);
+    return top + fy * (bot - top);
+}
+
+// ---------------------------------------------------------------------------
+// Hit attributes (position, facing normal, surface uv) for a closest hit.
+// ---------------------------------------------------------------------------
+struct Surf {
+    F3 pos, normal;
+    float u, v;
+    bool face;
+};
+
+__device__ void hit_attributes(const SceneDev &sc, const AvrtObject &ob, const HitRec &h, F3 o, F3 d, float time,
+                               Surf &s)
+{
+    s.u = 0.0f;
+    s.v = 0.0f;
+    const uint32_t shape = ob.shapeType;
+    if (shape == AVRT_SHAPE_SPHERE) {
+        const F3 P = f3(ob.x, ob.y, ob.z), P2 = f3(ob.x2, ob.y2, ob.z2);
+        const F3 centre = lerp(P, P2, time);
+        s.pos = o + d * h.t;
+        s.normal = (s.pos - centre) / ob.radius;
+        s.face = dot(d, s.normal) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        if (ob.textureID != AVRT_NO_TEXTURE) {
+            const float theta = acosf(-s.normal.y);
+            const float phi = atan2f(-s.normal.z, s.normal.x) + 3.14159265f;
+            const float a = phi / 6.2831853f - ob.rotation / 6.2831853f;
+            s.u = a - floorf(a);
+            s.v = theta / 3.14159265f;
+        }
+    } else if (shape == AVRT_SHAPE_QUAD || shape == AVRT_SHAPE_TRIANGLE) {
+        const F3 n = cross(f3(ob.u_x, ob.u_y, ob.u_z), f3(ob.v_x, ob.v_y, ob.v_z));
+        const float nl = dot(n, n);
+        s.pos = o + d * h.t;
+        s.normal = n / sqrtf(nl);
+        s.face = dot(d, s.normal) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        s.u = h.alpha;
+        s.v = h.beta;
+    } else if (shape == AVRT_SHAPE_MESH) {
+        const ObjGeom &g = sc.geom[h.obj];
+        const uint32_t tri_base = __float_as_uint(g.c[1]);
+        const uint32_t vbase = __float_as_uint(g.c[2]);
+        const uint32_t ibase = __float_as_uint(g.c[3]);
+        const uint32_t base = ibase + h.prim * 3u;
+        const AvrtVertex a = sc.verts[vbase + sc.indices[base + 0]];
+        const AvrtVertex b = sc.verts[vbase + sc.indices[base + 1]];
+        const AvrtVertex c = sc.verts[vbase + sc.indices[base + 2]];
+        const TriPre &tp = sc.tris[tri_base + h.prim];
+        const F3 n = cross(ld3(tp.e1), ld3(tp.e2));
+        F3 geo = n / sqrtf(dot(n, n));
+        const float w0 = 1.0f - h.alpha - h.beta;
+        const F3 shl = normalize(w0 * f3(a.nx, a.ny, a.nz) + h.alpha * f3(b.nx, b.ny, b.nz) + h.beta * f3(c.nx, c.ny, c.nz));
+        if (dot(geo, shl) < 0.0f) geo = -geo;
+        const float qx = g.b[0], qy = g.b[1], qz = g.b[2], qw = g.b[3];
+        s.pos = o + d * h.t;
+        s.normal = normalize(quat_rotate(qx, qy, qz, qw, shl));
+        s.face = dot(d, quat_rotate(qx, qy, qz, qw, geo)) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        s.u = w0 * a.u + h.alpha * b.u + h.beta * c.u;
+        s.v = w0 * a.v + h.alpha * b.v + h.beta * c.v;
+    } else {
+        const ObjGeom &g = sc.geom[h.obj];
+        BoxLocal r;
+        box_local(g, o, d, r);
+        const bool outside = (h.t == r.tin);
+        const F3 ln = outside ? r.en : r.ex;
+        s.pos = o + d * h.t;
+        const F3 lp = r.lo + r.ld * h.t;
+        const F3 half = f3(g.b[0], g.b[1], g.b[2]);
+        float uu, vv;
+        if (fabsf(ln.x) > 0.5f) {
+            uu = lp.z / half.z;
+            vv = lp.y / half.y;
+        } else if (fabsf(ln.y) > 0.5f) {
+            uu = lp.x / half.x;
+            vv = lp.z / half.z;
+        } else {
+            uu = lp.x / half.x;
+            vv = lp.y / half.y;
+        }
+        s.u = saturate(uu * 0.5f + 0.5f);
+        s.v = saturate(vv * 0.5f + 0.5f);
+        s.normal = rotate_y(ln, g.a[3], g.b[3]);
+        s.face = outside;
+        if (!s.face) s.normal = -s.normal;
+    }
+}
+
+// GetBounce direction (shared by both shaders).
+__device__ __forceinline__ bool get_bounce(const AvrtObject &ob, F3 d, const Surf &s, Rng &rng, F3 *bounce)
+{
+    *bounce = f3(0.0f, 0.0f, 0.0f);
+    switch (ob.colorType) {
+    case AVRT_LAMBERTIAN: {
+        const F3 rd = rng.direction();
+        *bounce = s.normal + rd;
+        if (dot(*bounce, *bounce) < 0.001f) *bounce = s.normal;
+        return true;
+    }
+    case AVRT_METAL: {
+        F3 b = normalize(reflect(d, s.normal));
+        const F3 rd = rng.direction();
+        b = b + rd * ob.fuzz;
+        *bounce = b;
+        return dot(b, s.normal) > 0.0f;
+    }
+    case AVRT_DIELECTRIC: {
+        const float ior = ob.refraction;
+        const float ri = s.face ? (1.0f / ior) : ior;
+        const F3 dir = normalize(d);
+        const float cosT = fminf(dot(-dir, s.normal), 1.0f);
+        const float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+        const float q = (1.0f - ri) / (1.0f + ri);
+        const float r0 = q * q;
+        const float m = 1.0f - cosT;
+        const float m5 = ((m * m) * (m * m)) * m;
+        const float refl = r0 + (1.0f - r0) * m5;
+        if (ri * sinT > 1.0f || refl > rng.next()) *bounce = reflect(dir, s.normal);
+        else *bounce = refract(dir, s.normal, ri);
+        return true;
+    }
+    case AVRT_ISOTROPIC:
+        *bounce = rng.direction();
+        return true;
+    default:
+        return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+struct PathBufs {
+    float4 *o;  // origin.xyz, time
+    float4 *d;  // direction.xyz, path id (bits)
+    float4 *t;  // throughput.xyz, rng state (bits)
+};
+
+struct ShadowBufs {
+    float4 *p;  // surface position.xyz, time
+    float4 *n;  // facing normal.xyz, path id
+    float4 *a;  // surface albedo.xyz, rng state at NEE
+    float4 *w;  // throughput.xyz
+};
+
+__global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t first_sample, uint32_t n, PathBufs out,
+                                                   float4 *Lpath)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= n) return;
+    const uint32_t P = fp.width * fp.height;
+    const uint32_t sidx = pid / P;
+    const uint32_t p = pid - sidx * P;
+    const uint32_t x = p % fp.width, y = p / fp.width;
+    const uint32_t batch = fp.batch + first_sample + sidx;
+    Rng rng{x + y * fp.width + batch * fp.width * fp.height + 1u};
+    const float jx = rng.next() - 0.5f;
+    const float jy = rng.next() - 0.5f;
+    const F3 p0 = ld3(fp.pixel0), pu = ld3(fp.pixelU), pv = ld3(fp.pixelV);
+    const F3 target = p0 + (static_cast<float>(x) + jx) * pu + (static_cast<float>(y) + jy) * pv;
+    F3 o = ld3(fp.source);
+    if (!(fp.defocus <= 0.0f)) {
+        float px, py;
+        while (true) {
+            px = rng.next() * 2.0f - 1.0f;
+            py = rng.next() * 2.0f - 1.0f;
+            if (sqrtf(px * px + py * py + 0.0f * 0.0f) < 1.0f) break;
+        }
+        o = ld3(fp.source) + (px * ld3(fp.du)) + (py * ld3(fp.dv));
+    }
+    const F3 d = normalize(target - o);
+    const float time = rng.next();
+    out.o[pid] = make_float4(o.x, o.y, o.z, time);
+    out.d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
+    out.t[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(rng.s));
+    Lpath[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <bool REF_ORDER>
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                   const float4 *__restrict__ rd, uint4 *__restrict__ hitA,
+                                                   float *__restrict__ hitB)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 O = ro[k];
+    const float4 D = rd[k];
+    HitRec h;
+    trace_closest<REF_ORDER, kStackLds, kBlock, kStackSpill>(sc, f3(O.x, O.y, O.z), f3(D.x, D.y, D.z), O.w, 0.001f,
+                                                             1000000.0f, lds_stack, h, nullptr);
+    hitA[k] = make_uint4(__float_as_uint(h.t), h.obj, h.prim, __float_as_uint(h.alpha));
+    hitB[k] = h.beta;
+}
+
+__device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)
+{
+    float4 L = Lpath[pid];
+    L.x = L.x + c.x;
+    L.y = L.y + c.y;
+    L.z = L.z + c.z;
+    Lpath[pid] = L;
+}
+
+// MODE 0 = path_trace.comp, MODE 1 = ray_trace.comp.hybrid.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, uint32_t n, uint32_t bounce,
+                                                  PathBufs in, const uint4 *__restrict__ hitA,
+                                                  const float *__restrict__ hitB, PathBufs out, ShadowBufs sh,
+                                                  float4 *Lpath, uint32_t *counters)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cont = false;
+    bool want_shadow = false;
+    F3 no = f3(0, 0, 0), nd = f3(0, 0, 0), ntp = f3(0, 0, 0);
+    float time = 0.0f;
+    uint32_t pid = 0, nrng = 0;
+    Surf s;
+    s.pos = f3(0, 0, 0);
+    s.normal = f3(0, 0, 0);
+    F3 albedo = f3(0, 0, 0), tp_at_hit = f3(0, 0, 0);
+    uint32_t rng_at_nee = 0;
+    if (k < n) {
+        const float4 O = in.o[k];
+        const float4 D = in.d[k];
+        const float4 T = in.t[k];
+        const uint4 HA = hitA[k];
+        const F3 o = f3(O.x, O.y, O.z), d = f3(D.x, D.y, D.z);
+        F3 tp = f3(T.x, T.y, T.z);
+        time = O.w;
+        pid = __float_as_uint(D.w);
+        Rng rng{__float_as_uint(T.w)};
+        HitRec h;
+        h.t = __uint_as_float(HA.x);
+        h.obj = HA.y;
+        h.prim = HA.z;
+        h.alpha = __uint_as_float(HA.w);
+        h.beta = hitB[k];
+        if (h.obj == ~0u) {
+            if (MODE == 0) {
+                const float t = 0.5f * (normalize(d).y + 1.0f);
+                add_radiance(Lpath, pid, tp * lerp(ld3(fp.horizon), ld3(fp.sky), t));
+            } else {
+                add_radiance(Lpath, pid, tp * ld3(fp.sky));
+            }
+        } else {
+            const AvrtObject ob = sc.objects[h.obj];
+            hit_attributes(sc, ob, h, o, d, time, s);
+            const F3 alb = f3(ob.r, ob.g, ob.b);
+            if (MODE == 0) {
+                F3 att = alb;
+                if (ob.textureID != AVRT_NO_TEXTURE) att = att * texture_color(sc, fp, ob.textureID, s.u, s.v);
+                if (ob.colorType == AVRT_DIFFUSE_LIGHT) add_radiance(Lpath, pid, tp * (att * ob.emission));
+                F3 b;
+                if (get_bounce(ob, d, s, rng, &b)) {
+                    no = s.pos;
+                    nd = b;
+                    ntp = tp * att;
+                    cont = (bounce + 1 < fp.depth);
+                }
+            } else {
+                if (ob.colorType == AVRT_DIFFUSE_LIGHT) {
+                    add_radiance(Lpath, pid, tp * (ob.emission * alb));
+                } else {
+                    F3 salb = alb;
+                    if (ob.textureID != AVRT_NO_TEXTURE) {
+                        const F3 texel = texture_color(sc, fp, ob.textureID, s.u, s.v);
+                        salb = texel * lerp(f3(1.0f, 1.0f, 1.0f), alb, ob.textureTint);
+                    }
+                    if (ob.colorType == AVRT_LAMBERTIAN && fp.num_lights > 0) {
+                        want_shadow = true;
+                        albedo = salb;
+                        tp_at_hit = tp;
+                        rng_at_nee = rng.s;
+                        rng.s = rng.s * fp.jumpA + fp.jumpC;
+                    }
+                    F3 sc_dir;
+                    if (get_bounce(ob, d, s, rng, &sc_dir)) {
+                        const F3 nd2 = normalize(sc_dir);
+                        const float sign = (dot(nd2, s.normal) < 0.0f) ? -1.0f : 1.0f;
+                        const F3 no2 = s.pos + s.normal * (0.001f * sign);
+                        if (ob.colorType != AVRT_DIELECTRIC) tp = tp * salb;
+                        bool alive = true;
+                        if (bounce > 0) {
+                            float p = fmaxf(tp.x, fmaxf(tp.y, tp.z));
+                            p = fminf(fmaxf(p, 0.1f), 1.0f);
+                            if (rng.next() > p) alive = false;
+                            else tp = f3(tp.x / p, tp.y / p, tp.z / p);
+                        }
+                        if (alive) {
+                            no = no2;
+                            nd = nd2;
+                            ntp = tp;
+                            cont = (bounce + 1 < fp.depth);
+                        }
+                    }
+                }
+            }
+        }
+        nrng = rng.s;
+    }
+    const uint32_t slot = wave_append(cont, &counters[0]);
+    if (cont) {
+        out.o[slot] = make_float4(no.x, no.y, no.z, time);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(pid));
+        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(nrng));
+    }
+    if (MODE == 1) {
+        const uint32_t ss = wave_append(want_shadow, &counters[1]);
+        if (want_shadow) {
+            sh.p[ss] = make_float4(s.pos.x, s.pos.y, s.pos.z, time);
+            sh.n[ss] = make_float4(s.normal.x, s.normal.y, s.normal.z, __uint_as_float(pid));
+            sh.a[ss] = make_float4(albedo.x, albedo.y, albedo.z, __uint_as_float(rng_at_nee));
+            sh.w[ss] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, 0.0f);
+        }
+    }
+}
+
+template <bool REF_ORDER>
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev sc, FrameParams fp, uint32_t n, ShadowBufs sh,
+                                                   float4 *Lpath, uint32_t *counters)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t traced = 0;
+    if (k < n) {
+        const float4 P4 = sh.p[k];
+        const float4 N4 = sh.n[k];
+        const float4 A4 = sh.a[k];
+        const float4 W4 = sh.w[k];
+        const F3 P = f3(P4.x, P4.y, P4.z), N = f3(N4.x, N4.y, N4.z), brdf = f3(A4.x, A4.y, A4.z);
+        const float time = P4.w;
+        const uint32_t pid = __float_as_uint(N4.w);
+        Rng rng{__float_as_uint(A4.w)};
+        F3 sum = f3(0.0f, 0.0f, 0.0f);
+        for (uint32_t i = 0; i < fp.num_lights; i++) {
+            const AvrtObject l = sc.objects[sc.lights[i]];
+            const F3 LP = f3(l.x, l.y, l.z);
+            F3 lp = LP;
+            float solid = 0.0f;
+            if (l.shapeType == AVRT_SHAPE_QUAD) {
+                const F3 lu = f3(l.u_x, l.u_y, l.u_z), lv = f3(l.v_x, l.v_y, l.v_z);
+                const float r1 = rng.next();
+                const float r2 = rng.next();
+                lp = LP + r1 * lu + r2 * lv;
+                const F3 cuv = cross(lu, lv);
+                const float area = length(cuv);
+                const F3 lnrm = cuv / fmaxf(area, 1e-8f);
+                F3 ts = P - lp;
+                const float d2 = dot(ts, ts);
+                ts = normalize(ts);
+                const float cl = fabsf(dot(lnrm, ts));
+                solid = (cl * area) / fmaxf(d2, 1e-8f);
+            } else if (l.shapeType == AVRT_SHAPE_SPHERE) {
+                const F3 tl = LP - P;
+                const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+                const float r2 = l.radius * l.radius;
+                if (d2 <= r2) {
+                    solid = 4.0f * 3.14159265f;
+                } else {
+                    const float ct = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+                    solid = 2.0f * 3.14159265f * (1.0f - ct);
+                }
+            } else {
+                const F3 tl = LP - P;
+                const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+                const float area = 4.0f * (l.half_x * l.half_y + l.half_y * l.half_z + l.half_z * l.half_x) / 6.0f;
+                solid = area / d2;
+            }
+            if (solid <= 0.0f) continue;
+            F3 dir = lp - P;
+            const float dist = length(dir);
+            if (dist <= 1e-6f) continue;
+            dir = dir / dist;
+            const float ct = dot(N, dir);
+            if (ct <= 0.0f) continue;
+            F3 so = P + (N * 0.001f);
+            bool occluded = false;
+            for (int step = 0; step < 4; step++) {
+                // Closest hit bounded by the light distance: equivalent to the
+                // reference's unbounded walk followed by its distance check.
+                const float bound = length(lp - so) - 0.001f;
+                HitRec hr;
+                traced++;
+                if (!trace_closest<REF_ORDER, kStackLds, kBlock, kStackSpill>(sc, so, dir, time, 0.001f, bound, lds_stack,
+                                                                           hr, nullptr))
+                    break;
+                if (hr.t >= bound) break;
+                const uint32_t ctype = sc.objects[hr.obj].colorType;
+                if (ctype == AVRT_DIFFUSE_LIGHT) break;
+                if (ctype != AVRT_DIELECTRIC) {
+                    occluded = true;
+                    break;
+                }
+                so = (so + dir * hr.t) + (dir * 0.001f);
+            }
+            if (!occluded) {
+                const F3 radiance = l.emission * f3(l.r, l.g, l.b);
+                sum = sum + brdf * radiance * ct * solid;
+            }
+        }
+        const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
+        if (luma > 8.0f) sum = sum * (8.0f / luma);
+        add_radiance(Lpath, pid, f3(W4.x, W4.y, W4.z) * sum);
+    }
+    const uint32_t tot = wave_sum(traced);
+    if (__lane_id() == 0 && tot) atomicAdd(&counters[2], tot);
+}
+
+__device__ __forceinline__ float aces(float x)
+{
+    return saturate((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f));
+}
+__device__ __forceinline__ float to_srgb(float c)
+{
+    c = saturate(c);
+    if (c <= 0.0031308f) return c * 12.92f;
+    return 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ unsigned char unorm8(float x) { return static_cast<unsigned char>(floorf(saturate(x) * 255.0f + 0.5f)); }
+__device__ __forceinline__ float round_half(float x) { return __half2float(__float2half(x)); }
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams fp, uint32_t first_sample, uint32_t ns,
+                                                    const float4 *Lpath, float4 *image, uchar4 *display)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t P = fp.width * fp.height;
+    if (p >= P) return;
+    float4 acc = image[p];
+    for (uint32_t s = 0; s < ns; s++) {
+        const uint32_t batch = fp.batch + first_sample + s;
+        const float4 c = Lpath[static_cast<size_t>(s) * P + p];
+        if (MODE == 0) {
+            const float px = batch > 0 ? acc.x : 0.0f;
+            const float py = batch > 0 ? acc.y : 0.0f;
+            const float pz = batch > 0 ? acc.z : 0.0f;
+            const float bf = static_cast<float>(batch), bn = static_cast<float>(batch + 1u);
+            acc.x = (px * bf + c.x) / bn;
+            acc.y = (py * bf + c.y) / bn;
+            acc.z = (pz * bf + c.z) / bn;
+        } else {
+            if (batch == 0) {
+                acc.x = c.x;
+                acc.y = c.y;
+                acc.z = c.z;
+            } else {
+                const float w = 1.0f / static_cast<float>(batch + 1u);
+                acc.x = acc.x + w * (c.x - acc.x);
+                acc.y = acc.y + w * (c.y - acc.y);
+                acc.z = acc.z + w * (c.z - acc.z);
+            }
+            if (fp.fp16) {
+                acc.x = round_half(acc.x);
+                acc.y = round_half(acc.y);
+                acc.z = round_half(acc.z);
+            }
+        }
+    }
+    acc.w = 1.0f;
+    image[p] = acc;
+    if (MODE == 1) {
+        const float e = fp.exposure;
+        display[p] = make_uchar4(unorm8(to_srgb(aces(acc.x * e))), unorm8(to_srgb(aces(acc.y * e))),
+                                 unorm8(to_srgb(aces(acc.z * e))), 255);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_display_only(FrameParams fp, const float4 *image, uchar4 *display)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fp.width * fp.height) return;
+    const float4 a = image[p];
+    const float e = fp.exposure;
+    display[p] = make_uchar4(unorm8(to_srgb(aces(a.x * e))), unorm8(to_srgb(aces(a.y * e))),
+                             unorm8(to_srgb(aces(a.z * e))), 255);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+struct avrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    AvrtCreateOptions opts{};
+    bool ref_order = false;
+
+    // geometry (host copies drive the per-mesh conversion)
+    std::vector<AvrtVertex> h_verts;
+    std::vector<uint32_t> h_indices;
+    std::vector<AvrtBVHNode> h_blas;
+    std::vector<Node2> h_blas2;
+    std::vector<TriPre> h_tris;
+    std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
+    std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase
+    bool geometry_loaded = false;
+    DevBuf d_verts, d_indices, d_blas2, d_tris;
+
+    // scene
+    std::vector<AvrtObject> h_objects;
+    std::vector<uint32_t> h_lights;
+    uint32_t nobj = 0, nlights = 0, nquad_lights = 0, ntlas2 = 0;
+    bool has_volume = false, has_tri_or_mesh = false;
+    bool scene_loaded = false;
+    DevBuf d_objects, d_geom, d_tlas2, d_lights;
+
+    // textures
+    DevBuf d_tex;
+    uint32_t tw = 0, th = 0, tl = 0;
+
+    // frame targets
+    DevBuf d_image, d_display;
+    uint32_t img_w = 0, img_h = 0;
+
+    // path buffers
+    DevBuf so[2], sd[2], st[2], hitA, hitB, Lpath, shp, shn, sha, shw, counters;
+    size_t cap_paths = 0;
+
+    AvrtStats stats{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_a = nullptr, ev_b = nullptr;
+};
+
+static std::string g_create_error;
+
+static int set_err(avrt_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPX(ctx, call)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return set_err((ctx), AVRT_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static int ensure(avrt_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return AVRT_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        return set_err(c, AVRT_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    b.bytes = bytes;
+    return AVRT_OK;
+}
+
+static void release(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// --- conversion of reference BVH arrays into Node2 trees ----------------------
+namespace {
+
+struct Converter {
+    const AvrtBVHNode *nodes;
+    uint32_t base;      // index of the tree's root in `nodes`
+    uint32_t avail;     // nodes available from base
+    std::vector<Node2> *out;
+    std::string err;
+
+    static void set_child(Node2 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
+    {
+        n.bx[c] = lo[0];
+        n.bx[c + 2] = hi[0];
+        n.by[c] = lo[1];
+        n.by[c + 2] = hi[1];
+        n.bz[c] = lo[2];
+        n.bz[c + 2] = hi[2];
+        n.ref[c] = ref;
+    }
+    static void set_empty(Node2 &n, int c)
+    {
+        const float far_pt[3] = {1e30f, 1e30f, 1e30f};
+        set_child(n, c, far_pt, far_pt, kRefInvalid);
+    }
+
+    // Reference for a leaf [first, first+count) whose box is lo/hi. Leaves
+    // over 127 primitives become small subtrees of the same box.
+    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref)
+    {
+        if (count == 0) {
+            err = "empty leaf";
+            return false;
+        }
+        if (first + count > kMaxLeafFirst) {
+            err = "leaf slot beyond 2^24";
+            return false;
+        }
+        if (count <= kMaxLeafCount) {
+            *ref = make_leaf(first, count);
+            return true;
+        }
+        const uint32_t me = static_cast<uint32_t>(out->size());
+        out->push_back(Node2{});
+        const uint32_t half = count / 2;
+        uint32_t r0, r1;
+        if (!leaf_ref(first, half, lo, hi, &r0)) return false;
+        if (!leaf_ref(first + half, count - half, lo, hi, &r1)) return false;
+        set_child((*out)[me], 0, lo, hi, r0);
+        set_child((*out)[me], 1, lo, hi, r1);
+        *ref = me;
+        return true;
+    }
+
+    bool ref_of(uint32_t local, int depth, uint32_t *ref)
+    {
+        if (local >= avail || depth > 64) {
+            err = "BVH child index out of range";
+            return false;
+        }
+        const AvrtBVHNode &n = nodes[base + local];
+        if (n.count > 0) {
+            const float lo[3] = {n.min_x, n.min_y, n.min_z};
+            const float hi[3] = {n.max_x, n.max_y, n.max_z};
+            if (n.left < 0) {
+                err = "negative leaf slot";
+                return false;
+            }
+            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref);
+        }
+        if (n.left < 0 || n.right < 0) {
+            err = "negative child index";
+            return false;
+        }
+        const uint32_t me = static_cast<uint32_t>(out->size());
+        out->push_back(Node2{});
+        uint32_t r[2];
+        const uint32_t kids[2] = {static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.right)};
+        for (int c = 0; c < 2; c++) {
+            if (!ref_of(kids[c], depth + 1, &r[c])) return false;
+        }
+        for (int c = 0; c < 2; c++) {
+            const AvrtBVHNode &ch = nodes[base + kids[c]];
+            const float lo[3] = {ch.min_x, ch.min_y, ch.min_z};
+            const float hi[3] = {ch.max_x, ch.max_y, ch.max_z};
+            set_child((*out)[me], c, lo, hi, r[c]);
+        }
+        return true;
+    }
+
+    // Appends the tree rooted at nodes[base] behind a super-root; returns the
+    // super-root's index.
+    bool convert(uint32_t *super)
+    {
+        const uint32_t me = static_cast<uint32_t>(out->size());
+        out->push_back(Node2{});
+        uint32_t r;
+        if (!ref_of(0, 0, &r)) return false;
+        const AvrtBVHNode &root = nodes[base];
+        const float lo[3] = {root.min_x, root.min_y, root.min_z};
+        const float hi[3] = {root.max_x, root.max_y, root.max_z};
+        set_child((*out)[me], 0, lo, hi, r);
+        set_empty((*out)[me], 1);
+        *super = me;
+        return true;
+    }
+};
+
+inline void hcross(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline float hdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+} // namespace
+
+// Prepares BLAS trees and triangle records for every mesh an instance uses.
+static int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ninst, bool *changed)
+{
+    *changed = false;
+    for (uint32_t i = 0; i < ninst; i++) {
+        const AvrtMeshInstance &m = inst[i];
+        if (m.triangleCount == 0) continue;
+        if (m.indexBase % 3 != 0) return set_err(c, AVRT_E_INVALID_ARG, "instance indexBase is not a multiple of 3");
+        if (static_cast<uint64_t>(m.indexBase) + 3ull * m.triangleCount > c->h_indices.size())
+            return set_err(c, AVRT_E_INVALID_ARG, "instance triangles beyond the uploaded index buffer");
+        if (m.blasBase >= c->h_blas.size()) return set_err(c, AVRT_E_INVALID_ARG, "instance blasBase beyond the BLAS buffer");
+        if (c->blas_root.find(m.blasBase) == c->blas_root.end()) {
+            Converter cv{c->h_blas.data(), m.blasBase, static_cast<uint32_t>(c->h_blas.size() - m.blasBase), &c->h_blas2, {}};
+            uint32_t super = 0;
+            if (!cv.convert(&super)) return set_err(c, AVRT_E_INVALID_ARG, "BLAS at " + std::to_string(m.blasBase) + ": " + cv.err);
+            c->blas_root[m.blasBase] = super;
+            *changed = true;
+        }
+        auto it = c->tri_vbase.find(m.indexBase);
+        if (it != c->tri_vbase.end()) {
+            if (it->second != m.vertexBase)
+                return set_err(c, AVRT_E_INVALID_ARG, "two instances share indexBase with different vertexBase");
+            continue;
+        }
+        c->tri_vbase[m.indexBase] = m.vertexBase;
+        const uint32_t tbase = m.indexBase / 3;
+        if (c->h_tris.size() < static_cast<size_t>(tbase) + m.triangleCount) c->h_tris.resize(static_cast<size_t>(tbase) + m.triangleCount);
+        for (uint32_t s = 0; s < m.triangleCount; s++) {
+            const uint32_t *ix = c->h_indices.data() + m.indexBase + 3 * s;
+            float p[3][3];
+            for (int k = 0; k < 3; k++) {
+                const uint64_t vi = static_cast<uint64_t>(m.vertexBase) + ix[k];
+                if (vi >= c->h_verts.size()) return set_err(c, AVRT_E_INVALID_ARG, "triangle index beyond the vertex buffer");
+                const AvrtVertex &v = c->h_verts[vi];
+                p[k][0] = v.px;
+                p[k][1] = v.py;
+                p[k][2] = v.pz;
+            }
+            TriPre t{};
+            const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+            const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+            float n[3];
+            hcross(e1, e2, n);
+            const float nl = hdot(n, n);
+            for (int k = 0; k < 3; k++) {
+                t.p0[k] = p[0][k];
+                t.e1[k] = e1[k];
+                t.e2[k] = e2[k];
+                t.w[k] = n[k] / nl;
+            }
+            t.d0 = hdot(n, p[0]);
+            t.valid = (nl < 1e-12f) ? 0u : 1u;
+            c->h_tris[tbase + s] = t;
+        }
+        *changed = true;
+    }
+    return AVRT_OK;
+}
+
+static int upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes)
+{
+    int rc = ensure(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPX(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+static SceneDev scene_dev(const avrt_ctx *c)
+{
+    SceneDev s{};
+    s.tlas = c->nobj ? static_cast<const Node2 *>(c->d_tlas2.p) : nullptr;
+    s.geom = static_cast<const ObjGeom *>(c->d_geom.p);
+    s.blas = static_cast<const Node2 *>(c->d_blas2.p);
+    s.tris = static_cast<const TriPre *>(c->d_tris.p);
+    s.objects = static_cast<const AvrtObject *>(c->d_objects.p);
+    s.verts = static_cast<const AvrtVertex *>(c->d_verts.p);
+    s.indices = static_cast<const uint32_t *>(c->d_indices.p);
+    s.lights = static_cast<const uint32_t *>(c->d_lights.p);
+    s.tex = static_cast<const uchar4 *>(c->d_tex.p);
+    s.nobj = c->nobj;
+    return s;
+}
+
+// Builds the per-slot geometry records the leaf tests read.
+static int build_geom(avrt_ctx *c, const AvrtObject *objects, uint32_t n, const AvrtMeshInstance *inst, uint32_t ninst,
+                      std::vector<ObjGeom> &geom)
+{
+    geom.assign(n, ObjGeom{});
+    c->has_volume = false;
+    c->has_tri_or_mesh = false;
+    for (uint32_t i = 0; i < n; i++) {
+        const AvrtObject &o = objects[i];
+        ObjGeom &g = geom[i];
+        if (o.colorType == AVRT_ISOTROPIC) c->has_volume = true;
+        uint32_t flags = o.shapeType & 0xFFu;
+        switch (o.shapeType) {
+        case AVRT_SHAPE_SPHERE:
+            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = o.radius;
+            g.b[0] = o.x2 - o.x; g.b[1] = o.y2 - o.y; g.b[2] = o.z2 - o.z;
+            flags |= kGeomValid;
+            break;
+        case AVRT_SHAPE_QUAD:
+        case AVRT_SHAPE_TRIANGLE: {
+            c->has_tri_or_mesh |= (o.shapeType == AVRT_SHAPE_TRIANGLE);
+            const float u[3] = {o.u_x, o.u_y, o.u_z}, v[3] = {o.v_x, o.v_y, o.v_z}, P[3] = {o.x, o.y, o.z};
+            float nn[3];
+            hcross(u, v, nn);
+            const float nl = hdot(nn, nn);
+            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = hdot(nn, P);
+            for (int k = 0; k < 3; k++) {
+                g.b[k] = u[k];
+                g.c[k] = v[k];
+                const float w = nn[k] / nl;
+                std::memcpy(&g.d[k], &w, 4);
+            }
+            if (!(nl < 1e-8f)) flags |= kGeomValid;
+            break;
+        }
+        case AVRT_SHAPE_MESH: {
+            c->has_tri_or_mesh = true;
+            if (o.instanceIndex >= ninst) return set_err(c, AVRT_E_INVALID_ARG, "mesh object instanceIndex out of range");
+            const AvrtMeshInstance &m = inst[o.instanceIndex];
+            g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = (m.scale == 0.0f) ? 1.0f : m.scale;
+            g.b[0] = m.qx; g.b[1] = m.qy; g.b[2] = m.qz; g.b[3] = m.qw;
+            if (m.triangleCount > 0) {
+                auto it = c->blas_root.find(m.blasBase);
+                if (it == c->blas_root.end()) return set_err(c, AVRT_E_STATE, "mesh BLAS not prepared");
+                const uint32_t words[4] = {it->second, m.indexBase / 3, m.vertexBase, m.indexBase};
+                std::memcpy(g.c, words, sizeof(words));
+                flags |= kGeomValid;
+            }
+            break;
+        }
+        default: {
+            const float s = std::sin(o.rotation), cs = std::cos(o.rotation);
+            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = s;
+            g.b[0] = o.half_x; g.b[1] = o.half_y; g.b[2] = o.half_z; g.b[3] = cs;
+            flags |= kGeomValid;
+            break;
+        }
+        }
+        g.d[3] = flags;
+    }
+    return AVRT_OK;
+}
+
+extern "C" {
+
+const char *avrt_version(void) { return "avrt 0.1 gfx950"; }
+
+int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
+{
+    if (out_ctx == nullptr) return set_err(nullptr, AVRT_E_INVALID_ARG, "out_ctx is null");
+    *out_ctx = nullptr;
+    AvrtCreateOptions o{};
+    if (opts) o = *opts;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(nullptr, AVRT_E_DEVICE, "no HIP device");
+    if (o.device < 0 || o.device >= ndev) return set_err(nullptr, AVRT_E_INVALID_ARG, "device ordinal out of range");
+    if (hipSetDevice(o.device) != hipSuccess) return set_err(nullptr, AVRT_E_DEVICE, "hipSetDevice failed");
+    avrt_ctx *c = new avrt_ctx();
+    c->device = o.device;
+    c->opts = o;
+    c->ref_order = (o.traversal_order != 0);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_err(nullptr, AVRT_E_DEVICE, "hipStreamCreate failed");
+    }
+    float lut[256];
+    for (int i = 0; i < 256; i++) {
+        const double v = i / 255.0;
+        lut[i] = static_cast<float>(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
+        avrt_destroy(c);
+        return set_err(nullptr, AVRT_E_DEVICE, "device initialisation failed");
+    }
+    *out_ctx = c;
+    return AVRT_OK;
+}
+
+int avrt_destroy(avrt_ctx *c)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects, &c->d_geom, &c->d_tlas2,
+                      &c->d_lights, &c->d_tex, &c->d_image, &c->d_display, &c->so[0], &c->so[1], &c->sd[0],
+                      &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn, &c->sha,
+                      &c->shw, &c->counters};
+    for (DevBuf *b : bufs) release(*b);
+    for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AVRT_OK;
+}
+
+const char *avrt_last_error(const avrt_ctx *c)
+{
+    if (c == nullptr) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int avrt_upload_geometry(avrt_ctx *c, const AvrtVertex *vertices, uint32_t vertex_count, const uint32_t *indices,
+                         uint32_t index_count, const AvrtBVHNode *blas, uint32_t blas_count)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if ((vertex_count && !vertices) || (index_count && !indices) || (blas_count && !blas))
+        return set_err(c, AVRT_E_INVALID_ARG, "null array with non-zero count");
+    if (index_count % 3 != 0) return set_err(c, AVRT_E_INVALID_ARG, "index count is not a multiple of 3");
+    HIPX(c, hipSetDevice(c->device));
+    c->h_verts.assign(vertices, vertices + vertex_count);
+    c->h_indices.assign(indices, indices + index_count);
+    c->h_blas.assign(blas, blas + blas_count);
+    c->h_blas2.clear();
+    c->h_tris.clear();
+    c->blas_root.clear();
+    c->tri_vbase.clear();
+    int rc = upload_vec(c, c->d_verts, vertices, sizeof(AvrtVertex) * vertex_count);
+    if (rc) return rc;
+    rc = upload_vec(c, c->d_indices, indices, sizeof(uint32_t) * index_count);
+    if (rc) return rc;
+    c->geometry_loaded = true;
+    c->scene_loaded = false;
+    return AVRT_OK;
+}
+
+int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_count, const AvrtBVHNode *tlas,
+                      uint32_t node_count, const uint32_t *light_ids, uint32_t light_count,
+                      const AvrtMeshInstance *instances, uint32_t instance_count)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if ((object_count && !objects) || (node_count && !tlas) || (light_count && !light_ids) ||
+        (instance_count && !instances))
+        return set_err(c, AVRT_E_INVALID_ARG, "null array with non-zero count");
+    if (object_count && node_count == 0) return set_err(c, AVRT_E_INVALID_ARG, "objects without a TLAS");
+    if (object_count >= kMaxLeafFirst) return set_err(c, AVRT_E_CAPACITY, "more than 2^24 objects");
+    for (uint32_t i = 0; i < light_count; i++)
+        if (light_ids[i] >= object_count) return set_err(c, AVRT_E_INVALID_ARG, "light slot out of range");
+    HIPX(c, hipSetDevice(c->device));
+    bool changed = false;
+    if (instance_count) {
+        if (!c->geometry_loaded) return set_err(c, AVRT_E_STATE, "mesh instances before avrt_upload_geometry");
+        int rc = prepare_meshes(c, instances, instance_count, &changed);
+        if (rc) return rc;
+    }
+    if (changed) {
+        int rc = upload_vec(c, c->d_blas2, c->h_blas2.data(), sizeof(Node2) * c->h_blas2.size());
+        if (rc) return rc;
+        rc = upload_vec(c, c->d_tris, c->h_tris.data(), sizeof(TriPre) * c->h_tris.size());
+        if (rc) return rc;
+    }
+    std::vector<ObjGeom> geom;
+    int rc = build_geom(c, objects, object_count, instances, instance_count, geom);
+    if (rc) return rc;
+    if (c->has_volume) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
+    std::vector<Node2> t2;
+    if (object_count) {
+        Converter cv{tlas, 0, node_count, &t2, {}};
+        uint32_t super = 0;
+        if (!cv.convert(&super)) return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + cv.err);
+        // leaf slots must address uploaded objects
+        for (uint32_t i = 0; i < node_count; i++)
+            if (tlas[i].count > 0 && static_cast<uint64_t>(tlas[i].left) + tlas[i].count > object_count)
+                return set_err(c, AVRT_E_INVALID_ARG, "TLAS leaf addresses slots beyond object_count");
+    }
+    rc = upload_vec(c, c->d_tlas2, t2.data(), sizeof(Node2) * t2.size());
+    if (rc) return rc;
+    rc = upload_vec(c, c->d_geom, geom.data(), sizeof(ObjGeom) * geom.size());
+    if (rc) return rc;
+    rc = upload_vec(c, c->d_objects, objects, sizeof(AvrtObject) * object_count);
+    if (rc) return rc;
+    rc = upload_vec(c, c->d_lights, light_ids, sizeof(uint32_t) * light_count);
+    if (rc) return rc;
+    c->h_objects.assign(objects, objects + object_count);
+    c->h_lights.assign(light_ids, light_ids + light_count);
+    c->nobj = object_count;
+    c->ntlas2 = static_cast<uint32_t>(t2.size());
+    c->nlights = light_count;
+    c->nquad_lights = 0;
+    for (uint32_t i = 0; i < light_count; i++)
+        if (objects[light_ids[i]].shapeType == AVRT_SHAPE_QUAD) c->nquad_lights++;
+    c->scene_loaded = true;
+    return AVRT_OK;
+}
+
+int avrt_upload_textures(avrt_ctx *c, const uint8_t *rgba8_srgb, uint32_t width, uint32_t height, uint32_t layers)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if (layers == 0) {
+        c->tw = c->th = c->tl = 0;
+        return AVRT_OK;
+    }
+    if (!rgba8_srgb || width == 0 || height == 0) return set_err(c, AVRT_E_INVALID_ARG, "bad texture arguments");
+    if (width > 8192 || height > 8192) return set_err(c, AVRT_E_CAPACITY, "texture layers are limited to 8192^2");
+    HIPX(c, hipSetDevice(c->device));
+    int rc = upload_vec(c, c->d_tex, rgba8_srgb, static_cast<size_t>(width) * height * layers * 4);
+    if (rc) return rc;
+    c->tw = width;
+    c->th = height;
+    c->tl = layers;
+    return AVRT_OK;
+}
+
+} // extern "C"
+
+// Camera basis on the host (ray_trace.comp.hybrid.hlsl:1347-1359).
+static void camera_params(const AvrtConfig &cfg, FrameParams &fp)
+{
+    auto sub = [](const float a[3], const float b[3], float o[3]) {
+        o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2];
+    };
+    auto nrm = [](float v[3]) {
+        const float inv = 1.0f / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
+    };
+    const float src[3] = {cfg.source_x, cfg.source_y, cfg.source_z};
+    const float tgt[3] = {cfg.target_x, cfg.target_y, cfg.target_z};
+    const float up[3] = {cfg.up_x, cfg.up_y, cfg.up_z};
+    float w[3], u[3], v[3];
+    sub(src, tgt, w);
+    nrm(w);
+    hcross(up, w, u);
+    nrm(u);
+    hcross(w, u, v);
+    const float rad = 0.0174532925f;
+    const float vh = 2.0f * std::tan((cfg.fov / 2.0f) * rad) * cfg.focus_dist;
+    const float vw = vh * static_cast<float>(cfg.width) / static_cast<float>(cfg.height);
+    float vu[3], vv[3];
+    for (int k = 0; k < 3; k++) {
+        vu[k] = vw * u[k];
+        vv[k] = vh * -v[k];
+        fp.pixelU[k] = vu[k] / static_cast<float>(cfg.width);
+        fp.pixelV[k] = vv[k] / static_cast<float>(cfg.height);
+    }
+    for (int k = 0; k < 3; k++)
+        fp.pixel0[k] = src[k] - cfg.focus_dist * w[k] - vu[k] / 2.0f - vv[k] / 2.0f + fp.pixelU[k] / 2.0f + fp.pixelV[k] / 2.0f;
+    const float dr = cfg.focus_dist * std::tan((cfg.defocus_angle / 2.0f) * rad);
+    for (int k = 0; k < 3; k++) {
+        fp.du[k] = u[k] * dr;
+        fp.dv[k] = v[k] * dr;
+        fp.source[k] = src[k];
+    }
+    fp.defocus = cfg.defocus_angle;
+    fp.sky[0] = cfg.sky_r; fp.sky[1] = cfg.sky_g; fp.sky[2] = cfg.sky_b;
+    fp.horizon[0] = cfg.horizon_r; fp.horizon[1] = cfg.horizon_g; fp.horizon[2] = cfg.horizon_b;
+    fp.exposure = cfg.exposure;
+    fp.width = cfg.width;
+    fp.height = cfg.height;
+    fp.batch = cfg.batch;
+    fp.depth = cfg.depth;
+}
+
+// Composition of n steps of seed = seed*A + C.
+static void rng_jump(uint32_t n, uint32_t *ja, uint32_t *jc)
+{
+    uint32_t a = 1u, c = 0u;
+    for (uint32_t i = 0; i < n; i++) {
+        a = a * 747796405u;
+        c = c * 747796405u + 2891336453u;
+    }
+    *ja = a;
+    *jc = c;
+}
+
+static inline uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+static int ensure_paths(avrt_ctx *c, size_t n)
+{
+    if (n <= c->cap_paths) return AVRT_OK;
+    int rc = 0;
+    for (int i = 0; i < 2; i++) {
+        if ((rc = ensure(c, c->so[i], n * 16))) return rc;
+        if ((rc = ensure(c, c->sd[i], n * 16))) return rc;
+        if ((rc = ensure(c, c->st[i], n * 16))) return rc;
+    }
+    if ((rc = ensure(c, c->hitA, n * 16))) return rc;
+    if ((rc = ensure(c, c->hitB, n * 4))) return rc;
+    if ((rc = ensure(c, c->Lpath, n * 16))) return rc;
+    if ((rc = ensure(c, c->shp, n * 16))) return rc;
+    if ((rc = ensure(c, c->shn, n * 16))) return rc;
+    if ((rc = ensure(c, c->sha, n * 16))) return rc;
+    if ((rc = ensure(c, c->shw, n * 16))) return rc;
+    if ((rc = ensure(c, c->counters, 64))) return rc;
+    c->cap_paths = n;
+    return AVRT_OK;
+}
+
+static int ensure_image(avrt_ctx *c, uint32_t w, uint32_t h)
+{
+    if (c->img_w == w && c->img_h == h && c->d_image.p) return AVRT_OK;
+    const size_t P = static_cast<size_t>(w) * h;
+    int rc = ensure(c, c->d_image, P * 16);
+    if (rc) return rc;
+    rc = ensure(c, c->d_display, P * 4);
+    if (rc) return rc;
+    HIPX(c, hipMemsetAsync(c->d_image.p, 0, P * 16, c->stream));
+    HIPX(c, hipMemsetAsync(c->d_display.p, 0, P * 4, c->stream));
+    c->img_w = w;
+    c->img_h = h;
+    return AVRT_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+static int validate_config(avrt_ctx *c, const AvrtConfig *cfg)
+{
+    if (cfg == nullptr) return set_err(c, AVRT_E_INVALID_ARG, "config is null");
+    if (cfg->width == 0 || cfg->height == 0) return set_err(c, AVRT_E_INVALID_ARG, "zero image size");
+    if ((c->opts.max_width && cfg->width > c->opts.max_width) || (c->opts.max_height && cfg->height > c->opts.max_height))
+        return set_err(c, AVRT_E_CAPACITY, "image larger than the context's maximum");
+    if (static_cast<uint64_t>(cfg->width) * cfg->height > (1ull << 28)) return set_err(c, AVRT_E_CAPACITY, "image too large");
+    if (!c->scene_loaded) return set_err(c, AVRT_E_STATE, "render before avrt_upload_scene");
+    return AVRT_OK;
+}
+
+// Launches the bounce loop for `ns` sample frames starting at sample s0.
+template <int MODE>
+static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns)
+{
+    const SceneDev sc = scene_dev(c);
+    const uint32_t P = fp.width * fp.height;
+    uint32_t n = P * ns;
+    uint32_t *counters = static_cast<uint32_t *>(c->counters.p);
+    PathBufs buf[2];
+    for (int i = 0; i < 2; i++)
+        buf[i] = PathBufs{static_cast<float4 *>(c->so[i].p), static_cast<float4 *>(c->sd[i].p), static_cast<float4 *>(c->st[i].p)};
+    ShadowBufs sh{static_cast<float4 *>(c->shp.p), static_cast<float4 *>(c->shn.p), static_cast<float4 *>(c->sha.p),
+                  static_cast<float4 *>(c->shw.p)};
+    float4 *Lpath = static_cast<float4 *>(c->Lpath.p);
+    HIPX(c, hipEventRecord(c->ev_a, c->stream));
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, fp, s0, n, buf[0], Lpath);
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipEventRecord(c->ev_b, c->stream));
+    HIPX(c, hipEventSynchronize(c->ev_b));
+    c->stats.ms_other += elapsed(c->ev_a, c->ev_b);
+    c->stats.paths += n;
+    int cur = 0;
+    for (uint32_t b = 0; b < fp.depth && n > 0; b++) {
+        if (b == 0) c->stats.rays_primary += n;
+        else c->stats.rays_bounce += n;
+        HIPX(c, hipMemsetAsync(counters, 0, 16, c->stream));
+        HIPX(c, hipEventRecord(c->ev_a, c->stream));
+        if (c->ref_order)
+            hipLaunchKernelGGL(k_extend<true>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, n, buf[cur].o, buf[cur].d,
+                               static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
+        else
+            hipLaunchKernelGGL(k_extend<false>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, n, buf[cur].o, buf[cur].d,
+                               static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipEventRecord(c->ev_b, c->stream));
+        c->stats.launches_extend++;
+        hipLaunchKernelGGL(k_shade<MODE>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
+                           static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
+                           Lpath, counters);
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipEventRecord(c->ev1, c->stream));
+        uint32_t host_counts[4] = {0, 0, 0, 0};
+        HIPX(c, hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, c->stream));
+        HIPX(c, hipStreamSynchronize(c->stream));
+        c->stats.ms_extend += elapsed(c->ev_a, c->ev_b);
+        c->stats.ms_shade += elapsed(c->ev_b, c->ev1);
+        const uint32_t nshadow = host_counts[1];
+        if (MODE == 1 && nshadow > 0) {
+            HIPX(c, hipEventRecord(c->ev_a, c->stream));
+            if (c->ref_order)
+                hipLaunchKernelGGL(k_shadow<true>, dim3(blocks_for(nshadow)), dim3(kBlock), 0, c->stream, sc, fp, nshadow, sh,
+                                   Lpath, counters);
+            else
+                hipLaunchKernelGGL(k_shadow<false>, dim3(blocks_for(nshadow)), dim3(kBlock), 0, c->stream, sc, fp, nshadow, sh,
+                                   Lpath, counters);
+            HIPX(c, hipGetLastError());
+            HIPX(c, hipEventRecord(c->ev_b, c->stream));
+            HIPX(c, hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, c->stream));
+            HIPX(c, hipStreamSynchronize(c->stream));
+            c->stats.ms_shadow += elapsed(c->ev_a, c->ev_b);
+            c->stats.rays_shadow += host_counts[2];
+        }
+        n = host_counts[0];
+        cur ^= 1;
+    }
+    HIPX(c, hipEventRecord(c->ev_a, c->stream));
+    hipLaunchKernelGGL(k_resolve<MODE>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, s0, ns, Lpath,
+                       static_cast<float4 *>(c->d_image.p), static_cast<uchar4 *>(c->d_display.p));
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipEventRecord(c->ev_b, c->stream));
+    HIPX(c, hipEventSynchronize(c->ev_b));
+    c->stats.ms_other += elapsed(c->ev_a, c->ev_b);
+    return AVRT_OK;
+}
+
+static int frame_params(avrt_ctx *c, const AvrtConfig *cfg, FrameParams &fp)
+{
+    std::memset(&fp, 0, sizeof(fp));
+    camera_params(*cfg, fp);
+    fp.spp = cfg->samples ? cfg->samples : 1u;
+    fp.num_lights = c->nlights;
+    fp.quad_lights = c->nquad_lights;
+    rng_jump(2u * c->nquad_lights, &fp.jumpA, &fp.jumpC);
+    fp.fp16 = c->opts.fp16_accum ? 1u : 0u;
+    fp.has_texture = c->tl ? 1u : 0u;
+    fp.tex_w = c->tw;
+    fp.tex_h = c->th;
+    fp.tex_layers = c->tl;
+    return AVRT_OK;
+}
+
+extern "C" {
+
+int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    int rc = validate_config(c, cfg);
+    if (rc) return rc;
+    if (mode != AVRT_MODE_PATH_TRACE && mode != AVRT_MODE_RAY_TRACE) return set_err(c, AVRT_E_INVALID_ARG, "unknown mode");
+    if (mode == AVRT_MODE_PATH_TRACE && c->has_tri_or_mesh)
+        return set_err(c, AVRT_E_UNSUPPORTED,
+                       "path_trace mode has no triangle or mesh intersector (the reference's HitObject falls through to HitBox)");
+    HIPX(c, hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    rc = ensure_image(c, cfg->width, cfg->height);
+    if (rc) return rc;
+    FrameParams fp;
+    frame_params(c, cfg, fp);
+    const uint32_t P = cfg->width * cfg->height;
+    HIPX(c, hipEventRecord(c->ev0, c->stream));
+    if (cfg->displayOnly != 0 && mode == AVRT_MODE_RAY_TRACE) {
+        hipLaunchKernelGGL(k_display_only, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp,
+                           static_cast<const float4 *>(c->d_image.p), static_cast<uchar4 *>(c->d_display.p));
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipStreamSynchronize(c->stream));
+        return AVRT_OK;
+    }
+    const uint32_t S = fp.spp;
+    size_t cap = c->opts.max_paths ? c->opts.max_paths : static_cast<size_t>(P) * S;
+    if (cap < P) cap = P;
+    const uint32_t per_chunk = static_cast<uint32_t>(std::min<size_t>(S, cap / P));
+    rc = ensure_paths(c, static_cast<size_t>(P) * per_chunk);
+    if (rc) return rc;
+    for (uint32_t s0 = 0; s0 < S; s0 += per_chunk) {
+        const uint32_t ns = std::min(per_chunk, S - s0);
+        rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk<0>(c, fp, s0, ns) : run_chunk<1>(c, fp, s0, ns);
+        if (rc) return rc;
+    }
+    HIPX(c, hipEventRecord(c->ev1, c->stream));
+    HIPX(c, hipEventSynchronize(c->ev1));
+    c->stats.ms_total = elapsed(c->ev0, c->ev1);
+    return AVRT_OK;
+}
+
+int avrt_get_stats(const avrt_ctx *c, AvrtStats *out)
+{
+    if (c == nullptr || out == nullptr) return AVRT_E_INVALID_ARG;
+    *out = c->stats;
+    return AVRT_OK;
+}
+
+int avrt_read_image(avrt_ctx *c, float *dst, size_t bytes)
+{
+    if (c == nullptr || dst == nullptr) return AVRT_E_INVALID_ARG;
+    const size_t need = static_cast<size_t>(c->img_w) * c->img_h * 16;
+    if (need == 0) return set_err(c, AVRT_E_STATE, "nothing rendered yet");
+    if (bytes < need) return set_err(c, AVRT_E_CAPACITY, "destination too small");
+    HIPX(c, hipSetDevice(c->device));
+    HIPX(c, hipMemcpyAsync(dst, c->d_image.p, need, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+int avrt_read_display(avrt_ctx *c, uint8_t *dst, size_t bytes)
+{
+    if (c == nullptr || dst == nullptr) return AVRT_E_INVALID_ARG;
+    const size_t need = static_cast<size_t>(c->img_w) * c->img_h * 4;
+    if (need == 0) return set_err(c, AVRT_E_STATE, "nothing rendered yet");
+    if (bytes < need) return set_err(c, AVRT_E_CAPACITY, "destination too small");
+    HIPX(c, hipSetDevice(c->device));
+    HIPX(c, hipMemcpyAsync(dst, c->d_display.p, need, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object, uint32_t *hit_prim, float *hit_t)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    int rc = validate_config(c, cfg);
+    if (rc) return rc;
+    HIPX(c, hipSetDevice(c->device));
+    FrameParams fp;
+    frame_params(c, cfg, fp);
+    const uint32_t P = cfg->width * cfg->height;
+    rc = ensure_paths(c, P);
+    if (rc) return rc;
+    PathBufs buf{static_cast<float4 *>(c->so[0].p), static_cast<float4 *>(c->sd[0].p), static_cast<float4 *>(c->st[0].p)};
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, 0u, P, buf,
+                       static_cast<float4 *>(c->Lpath.p));
+    HIPX(c, hipGetLastError());
+    const SceneDev sc = scene_dev(c);
+    if (c->ref_order)
+        hipLaunchKernelGGL(k_extend<true>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, sc, P, buf.o, buf.d,
+                           static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, sc, P, buf.o, buf.d,
+                           static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
+    HIPX(c, hipGetLastError());
+    std::vector<uint4> h(P);
+    HIPX(c, hipMemcpyAsync(h.data(), c->hitA.p, sizeof(uint4) * P, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    for (uint32_t p = 0; p < P; p++) {
+        float t;
+        std::memcpy(&t, &h[p].x, 4);
+        const bool hit = h[p].y != ~0u;
+        if (hit_object) hit_object[p] = h[p].y;
+        if (hit_prim) hit_prim[p] = hit ? h[p].z : ~0u;
+        if (hit_t) hit_t[p] = hit ? t : 0.0f;
+    }
+    return AVRT_OK;
+}
+
+} // extern "C"
+
+// Device-side scene rebuild entry points (implemented in a later step).
+extern "C" {
+int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *, uint32_t, const AvrtMeshInstance *, uint32_t,
+                             const AvrtAABB *)
+{
+    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
+}
+int avrt_scene_step(avrt_ctx *c, const float *, float) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+int avrt_scene_read(avrt_ctx *c, AvrtBVHNode *, uint32_t *, AvrtObject *, AvrtMeshInstance *, uint32_t *)
+{
+    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
+}
+int avrt_refit_blas(avrt_ctx *c) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *, uint32_t, uint32_t)
+{
+    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
+}
+int avrt_read_blas(avrt_ctx *c, AvrtBVHNode *, uint32_t) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+}

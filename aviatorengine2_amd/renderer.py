@@ -1,0 +1,123 @@
+"""Python mirror of the reference's Renderer interface (renderer/renderer.h:
+325-418) over the C ABI of libavrt.so.
+
+    Renderer.Initialize(settings)          -> avrt_create
+    Renderer.LoadScene(packed)             -> avrt_upload_geometry / _textures / _scene
+                                              (renderer.cpp:653-704)
+    Renderer.UploadScene(...)              -> avrt_upload_scene (renderer.cpp:931-1027)
+    Renderer.RenderFrame(config, mode)     -> avrt_render (renderer.cpp:1332-1588)
+    Renderer.ReadImage / ReadDisplay       -> avrt_read_image / avrt_read_display
+    Renderer.Shutdown()                    -> avrt_destroy
+
+Errors come back as AvrtError with the library's message; there is no CPU
+fallback: without libavrt.so (and a GPU) every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .abi import AvrtCreateOptions, AvrtError, AvrtStats, ptr
+
+
+class Renderer:
+    def __init__(self):
+        self.lib = None
+        self.ctx = None
+        self.width = 0
+        self.height = 0
+
+    # -- lifecycle ---------------------------------------------------------
+    def Initialize(self, device=0, max_width=0, max_height=0, max_paths=0, fp16_accum=False, reference_order=False):
+        self.lib = abi.load_library()
+        opts = AvrtCreateOptions()
+        opts.device = device
+        opts.max_width = max_width
+        opts.max_height = max_height
+        opts.max_paths = max_paths
+        opts.fp16_accum = 1 if fp16_accum else 0
+        opts.traversal_order = 1 if reference_order else 0
+        ctx = ctypes.c_void_p()
+        rc = self.lib.avrt_create(ctypes.byref(opts), ctypes.byref(ctx))
+        if rc:
+            raise AvrtError(rc, self.lib.avrt_last_error(None).decode())
+        self.ctx = ctx
+        return self
+
+    def Shutdown(self):
+        if self.ctx is not None:
+            self.lib.avrt_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.Shutdown()
+
+    def _check(self, rc, what):
+        if rc:
+            raise AvrtError(rc, f"{what}: {self.lib.avrt_last_error(self.ctx).decode()}")
+
+    # -- uploads -------------------------------------------------------------
+    def UploadMeshGeometry(self, vertices, indices, blas):
+        v = np.ascontiguousarray(vertices, abi.VERTEX_DTYPE)
+        i = np.ascontiguousarray(indices, np.uint32)
+        b = np.ascontiguousarray(blas, abi.NODE_DTYPE)
+        self._check(self.lib.avrt_upload_geometry(self.ctx, ptr(v), len(v), ptr(i), len(i), ptr(b), len(b)),
+                    "avrt_upload_geometry")
+
+    def UploadScene(self, objects, nodes, lights, instances):
+        o = np.ascontiguousarray(objects, abi.OBJECT_DTYPE)
+        n = np.ascontiguousarray(nodes, abi.NODE_DTYPE)
+        l = np.ascontiguousarray(lights, np.uint32)
+        m = np.ascontiguousarray(instances, abi.INSTANCE_DTYPE)
+        self._check(self.lib.avrt_upload_scene(self.ctx, ptr(o), len(o), ptr(n), len(n), ptr(l), len(l), ptr(m),
+                                               len(m)), "avrt_upload_scene")
+
+    def UploadTextures(self, texture):
+        if texture is None:
+            self._check(self.lib.avrt_upload_textures(self.ctx, None, 0, 0, 0), "avrt_upload_textures")
+            return
+        t = np.ascontiguousarray(texture, np.uint8)
+        layers, h, w, _ = t.shape
+        self._check(self.lib.avrt_upload_textures(self.ctx, ptr(t), w, h, layers), "avrt_upload_textures")
+
+    def LoadScene(self, packed):
+        self.UploadMeshGeometry(packed.vertices, packed.indices, packed.blas)
+        self.UploadTextures(packed.texture)
+        self.UploadScene(packed.objects, packed.tlas, packed.lights, packed.instances)
+
+    # -- frames --------------------------------------------------------------
+    def RenderFrame(self, config, mode):
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        self.width, self.height = int(c["width"][0]), int(c["height"][0])
+        self._check(self.lib.avrt_render(self.ctx, c.ctypes.data_as(ctypes.c_void_p), int(mode)), "avrt_render")
+        return self.Stats()
+
+    def Stats(self) -> dict:
+        s = AvrtStats()
+        self._check(self.lib.avrt_get_stats(self.ctx, ctypes.byref(s)), "avrt_get_stats")
+        return s.as_dict()
+
+    def ReadImage(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.float32)
+        self._check(self.lib.avrt_read_image(self.ctx, ptr(out), out.nbytes), "avrt_read_image")
+        return out
+
+    def ReadDisplay(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        self._check(self.lib.avrt_read_display(self.ctx, ptr(out), out.nbytes), "avrt_read_display")
+        return out
+
+    def TracePrimary(self, config):
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        P = int(c["width"][0]) * int(c["height"][0])
+        obj = np.empty(P, np.uint32)
+        prim = np.empty(P, np.uint32)
+        t = np.empty(P, np.float32)
+        self._check(self.lib.avrt_trace_primary(self.ctx, c.ctypes.data_as(ctypes.c_void_p), ptr(obj), ptr(prim), ptr(t)),
+                    "avrt_trace_primary")
+        return obj, prim, t

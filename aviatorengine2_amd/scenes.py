@@ -1,0 +1,645 @@
+"""Seeded synthetic scenes for the ray-tracing path, and their packing into the
+reference ABI arrays.
+
+Scene authoring in the reference (scene/scene.h, scene/scene_builders.cpp)
+uses SDL_randf and loaded meshes; the workloads here are defined instead by
+seeded PCG32 streams:
+
+  sphere_field()   Ray Tracing in One Weekend's final scene: 488 analytic
+                   spheres, path_trace semantics.
+  icosphere_tris() level-6 noise-displaced icosphere as 81,920 Triangle
+                   objects + 2 ground triangles + one emissive quad, single
+                   level BVH, ray_trace semantics.
+  instanced()      two-level scene: instances of four seeded prototype meshes
+                   with random rotation, scale and translation, one procedural
+                   Perlin texture, one emissive sphere.
+
+Packing follows the reference host code:
+  MeshLibrary::Add        scene/mesh_library.h:91-159  (local vertices, BLAS
+                          per mesh, index buffer in BLAS slot order)
+  GatherRenderables       scene/shapes.h:379-426 (spheres, quads, boxes,
+                          triangles, meshes) and the *Bounds functions
+                          scene/shapes.h:71-145
+  RebuildAcceleration     renderer/renderer.cpp:1087-1130 (BuildBVH over the
+                          renderable bounds, objects in slot order)
+  PackObjects             renderer/renderer.cpp:815-929 (instances in slot
+                          order, instanceIndex = rank among meshes)
+  LightIndices            renderer/renderer.cpp:803-813
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .abi import (AABB_DTYPE, CONFIG_DTYPE, DIELECTRIC, DIFFUSE_LIGHT, INSTANCE_DTYPE, LAMBERTIAN, MESH, METAL,
+                  NODE_DTYPE, NO_TEXTURE, OBJECT_DTYPE, QUAD, SPHERE, TRIANGLE, VERTEX_DTYPE)
+
+M32 = 0xFFFFFFFF
+M64 = 0xFFFFFFFFFFFFFFFF
+DEFAULT_TEXTURE_TINT = 0.25  # kDefaultTextureTint, scene/material.h:44
+
+
+class Pcg32:
+    """PCG32 (XSH-RR), O'Neill 2014."""
+
+    def __init__(self, seed: int, seq: int = 54):
+        self.state = 0
+        self.inc = ((seq << 1) | 1) & M64
+        self.next_u32()
+        self.state = (self.state + seed) & M64
+        self.next_u32()
+
+    def next_u32(self) -> int:
+        old = self.state
+        self.state = (old * 6364136223846793005 + self.inc) & M64
+        xs = (((old >> 18) ^ old) >> 27) & M32
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & M32
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        u = (self.next_u32() >> 8) * (1.0 / 16777216.0)
+        return lo + (hi - lo) * u
+
+    def normal(self) -> float:
+        u1 = 1.0 - self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+    def quaternion(self):
+        """Uniform random rotation (Shoemake 1992): (x, y, z, w)."""
+        u1, u2, u3 = self.uniform(), self.uniform(), self.uniform()
+        a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+        return (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+                b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+
+
+# ---------------------------------------------------------------------------
+# Noise
+# ---------------------------------------------------------------------------
+def _hash_u32(*ints):
+    h = np.uint32(0x9E3779B9)
+    for k, v in zip((0x8DA6B343, 0xD8163841, 0xCB1AB31F, 0x165667B1), ints):
+        h = h ^ (np.asarray(v).astype(np.uint32) * np.uint32(k))
+        h = h ^ (h >> np.uint32(15))
+        h = h * np.uint32(0x2C1B3C6D)
+    h = h ^ (h >> np.uint32(12))
+    h = h * np.uint32(0x297A2D39)
+    h = h ^ (h >> np.uint32(15))
+    return h
+
+
+def value_noise3(p: np.ndarray, seed: int) -> np.ndarray:
+    """Trilinear value noise in [-1, 1] on the integer lattice, smoothstep
+    weights, lattice values from an integer hash."""
+    with np.errstate(over="ignore"):
+        f = np.floor(p)
+        i = f.astype(np.int64)
+        t = p - f
+        s = t * t * (3.0 - 2.0 * t)
+        out = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    h = _hash_u32(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, np.full(p.shape[0], seed))
+                    val = (h & np.uint32(0xFFFFFF)).astype(np.float64) / 8388608.0 - 1.0
+                    w = (s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) * (s[:, 2] if dz else 1 - s[:, 2])
+                    out += w * val
+    return out
+
+
+def fbm3(p: np.ndarray, seed: int, octaves: int = 3, freq: float = 2.0) -> np.ndarray:
+    total, norm, amp = np.zeros(p.shape[0]), 0.0, 1.0
+    for o in range(octaves):
+        total += amp * value_noise3(p * freq * (2 ** o), seed + 7919 * o)
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+def perlin_texture(size: int, seed: int, octaves: int = 4, cells: int = 8) -> np.ndarray:
+    """Tileable 2D gradient (Perlin) noise, `octaves` octaves, mapped to an
+    RGBA8 sRGB albedo texture of size x size. Returns uint8 (size, size, 4)."""
+    y, x = np.meshgrid(np.arange(size, dtype=np.float32) / size, np.arange(size, dtype=np.float32) / size, indexing="ij")
+    acc = np.zeros((size, size), np.float32)
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        period = cells * (2 ** o)
+        px, py = x * period, y * period
+        ix, iy = np.floor(px).astype(np.int64), np.floor(py).astype(np.int64)
+        fx, fy = (px - ix).astype(np.float32), (py - iy).astype(np.float32)
+
+        def grad(gx, gy, dx, dy):
+            h = _hash_u32(gx % period, gy % period, np.int64(seed + 31 * o))
+            ang = (h & np.uint32(0xFFFF)).astype(np.float32) * np.float32(2 * math.pi / 65536.0)
+            return np.cos(ang) * dx + np.sin(ang) * dy
+
+        n00 = grad(ix, iy, fx, fy)
+        n10 = grad(ix + 1, iy, fx - 1, fy)
+        n01 = grad(ix, iy + 1, fx, fy - 1)
+        n11 = grad(ix + 1, iy + 1, fx - 1, fy - 1)
+        u = fx * fx * fx * (fx * (fx * 6 - 15) + 10)
+        v = fy * fy * fy * (fy * (fy * 6 - 15) + 10)
+        nx0 = n00 + u * (n10 - n00)
+        nx1 = n01 + u * (n11 - n01)
+        acc += amp * (nx0 + v * (nx1 - nx0))
+        norm += amp
+        amp *= 0.5
+        del n00, n10, n01, n11, px, py, ix, iy, fx, fy, u, v, nx0, nx1
+    n = np.clip(acc / norm * 1.4 + 0.5, 0.0, 1.0)
+    rgb = np.stack([0.25 + 0.7 * n, 0.2 + 0.6 * n * n, 0.55 - 0.35 * n], axis=-1)
+    rgb = np.clip(rgb, 0.0, 1.0)
+    enc = np.where(rgb <= 0.0031308, rgb * 12.92, 1.055 * np.power(rgb, 1 / 2.4) - 0.055)
+    out = np.empty((size, size, 4), np.uint8)
+    out[..., :3] = np.round(enc * 255.0).astype(np.uint8)
+    out[..., 3] = 255
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Meshes: positions (n,3) float32, normals (n,3), uvs (n,2), tris (m,3) uint32
+# ---------------------------------------------------------------------------
+@dataclass
+class Mesh:
+    pos: np.ndarray
+    nrm: np.ndarray
+    uv: np.ndarray
+    tri: np.ndarray
+
+
+def _normalize_rows(v):
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere by `level` midpoint subdivisions: 10*4^L+2 vertices,
+    20*4^L faces, counter-clockwise seen from outside."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    V = _normalize_rows(V)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+                  [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        nv = V.shape[0]
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]], axis=0)
+        es = np.sort(e, axis=1)
+        key = es[:, 0] * nv + es[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a, b = uniq // nv, uniq % nv
+        mids = _normalize_rows((V[a] + V[b]) * 0.5)
+        m = nv + inv.reshape(3, -1).T  # columns: mid01, mid12, mid20
+        V = np.concatenate([V, mids], axis=0)
+        F = np.concatenate([
+            np.stack([F[:, 0], m[:, 0], m[:, 2]], 1),
+            np.stack([F[:, 1], m[:, 1], m[:, 0]], 1),
+            np.stack([F[:, 2], m[:, 2], m[:, 1]], 1),
+            np.stack([m[:, 0], m[:, 1], m[:, 2]], 1)], axis=0)
+    return V, F
+
+
+def sphere_uv(n: np.ndarray) -> np.ndarray:
+    u = 0.5 + np.arctan2(n[:, 2], n[:, 0]) / (2 * math.pi)
+    v = 0.5 - np.arcsin(np.clip(n[:, 1], -1, 1)) / math.pi
+    return np.stack([u, v], 1)
+
+
+def _vertex_normals(pos: np.ndarray, tri: np.ndarray) -> np.ndarray:
+    a, b, c = pos[tri[:, 0]], pos[tri[:, 1]], pos[tri[:, 2]]
+    fn = np.cross(b - a, c - a)
+    n = np.zeros_like(pos)
+    for k in range(3):
+        np.add.at(n, tri[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    ln[ln == 0] = 1
+    return n / ln
+
+
+def mesh_icosphere(level: int, radius: float) -> Mesh:
+    V, F = icosphere(level)
+    return Mesh((V * radius).astype(np.float32), V.astype(np.float32), sphere_uv(V).astype(np.float32),
+                F.astype(np.uint32))
+
+
+def mesh_cube_grid(n: int, half: float) -> Mesh:
+    """Cube [-half, half]^3 with every face split into n x n quads."""
+    pos, nrm, uv, tri = [], [], [], []
+    g = np.linspace(-1.0, 1.0, n + 1)
+    gu, gv = np.meshgrid(g, g, indexing="ij")
+    faces = [((1, 0, 0), (0, 0, -1), (0, 1, 0)), ((-1, 0, 0), (0, 0, 1), (0, 1, 0)),
+             ((0, 1, 0), (1, 0, 0), (0, 0, -1)), ((0, -1, 0), (1, 0, 0), (0, 0, 1)),
+             ((0, 0, 1), (1, 0, 0), (0, 1, 0)), ((0, 0, -1), (-1, 0, 0), (0, 1, 0))]
+    base = 0
+    for nn, uu, vv in faces:
+        nn, uu, vv = np.array(nn, float), np.array(uu, float), np.array(vv, float)
+        p = nn[None, :] + gu.reshape(-1, 1) * uu[None, :] + gv.reshape(-1, 1) * vv[None, :]
+        pos.append(p * half)
+        nrm.append(np.repeat(nn[None, :], p.shape[0], 0))
+        uv.append(np.stack([(gu.reshape(-1) + 1) / 2, (gv.reshape(-1) + 1) / 2], 1))
+        i = np.arange(n)
+        ii, jj = np.meshgrid(i, i, indexing="ij")
+        v00 = (ii * (n + 1) + jj).reshape(-1) + base
+        v10 = v00 + (n + 1)
+        v01 = v00 + 1
+        v11 = v10 + 1
+        tri.append(np.stack([v00, v10, v11], 1))
+        tri.append(np.stack([v00, v11, v01], 1))
+        base += (n + 1) * (n + 1)
+    return Mesh(np.concatenate(pos).astype(np.float32), np.concatenate(nrm).astype(np.float32),
+                np.concatenate(uv).astype(np.float32), np.concatenate(tri).astype(np.uint32))
+
+
+def mesh_torus(nu: int, nv: int, R: float, r: float) -> Mesh:
+    a = np.arange(nu) * (2 * math.pi / nu)
+    b = np.arange(nv) * (2 * math.pi / nv)
+    A, B = np.meshgrid(a, b, indexing="ij")
+    A, B = A.reshape(-1), B.reshape(-1)
+    ca, sa, cb, sb = np.cos(A), np.sin(A), np.cos(B), np.sin(B)
+    pos = np.stack([(R + r * cb) * ca, r * sb, (R + r * cb) * sa], 1)
+    nrm = np.stack([cb * ca, sb, cb * sa], 1)
+    uv = np.stack([A / (2 * math.pi), B / (2 * math.pi)], 1)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    i, j = i.reshape(-1), j.reshape(-1)
+    v00 = i * nv + j
+    v10 = ((i + 1) % nu) * nv + j
+    v01 = i * nv + (j + 1) % nv
+    v11 = ((i + 1) % nu) * nv + (j + 1) % nv
+    tri = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)])
+    return Mesh(pos.astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32), tri.astype(np.uint32))
+
+
+def mesh_noise_plane(n: int, half: float, amp: float, seed: int) -> Mesh:
+    g = np.linspace(-1.0, 1.0, n + 1)
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    X, Z = X.reshape(-1), Z.reshape(-1)
+    Y = amp * fbm3(np.stack([X * 2.0, np.zeros_like(X), Z * 2.0], 1), seed) / half
+    pos = np.stack([X, Y, Z], 1) * half
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v00 = (i * (n + 1) + j).reshape(-1)
+    v10 = v00 + (n + 1)
+    v01 = v00 + 1
+    v11 = v10 + 1
+    tri = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)]).astype(np.uint32)
+    nrm = _vertex_normals(pos, tri)
+    uv = np.stack([(X + 1) / 2, (Z + 1) / 2], 1)
+    return Mesh(pos.astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32), tri)
+
+
+# ---------------------------------------------------------------------------
+# Scene description
+# ---------------------------------------------------------------------------
+@dataclass
+class SceneSpec:
+    """Authoring-order scene: objects in GatherRenderables order, one
+    instance record per mesh object (same order), the meshes themselves, a
+    texture array and the Config."""
+    name: str
+    mode: int
+    objects: np.ndarray                      # OBJECT_DTYPE, authoring order
+    meshes: list = field(default_factory=list)
+    instance_mesh: list = field(default_factory=list)    # mesh id per mesh object
+    instances: np.ndarray | None = None      # INSTANCE_DTYPE transform part (px..qw) per mesh object
+    texture: np.ndarray | None = None        # uint8 (layers, h, w, 4)
+    config: np.ndarray | None = None         # CONFIG_DTYPE record
+    seed: int = 0
+    velocities: np.ndarray | None = None     # (n_mesh_objects, 6) float32 for the rigid-body motion
+
+
+def make_config(width, height, spp, depth, source, target, fov, focus, defocus, sky, horizon=(1, 1, 1),
+                exposure=1.0) -> np.ndarray:
+    c = np.zeros((), CONFIG_DTYPE)
+    c["source_x"], c["source_y"], c["source_z"] = source
+    c["target_x"], c["target_y"], c["target_z"] = target
+    c["fov"], c["focus_dist"], c["defocus_angle"] = fov, focus, defocus
+    c["up_x"], c["up_y"], c["up_z"] = 0.0, 1.0, 0.0
+    c["oof"] = 0.6
+    c["sky_r"], c["sky_g"], c["sky_b"] = sky
+    c["horizon_r"], c["horizon_g"], c["horizon_b"] = horizon
+    c["width"], c["height"] = width, height
+    c["samples"], c["batches"], c["batch"], c["depth"] = spp, 1, 0, depth
+    c["exposure"] = exposure
+    c["zNear"], c["zFar"] = 0.1, 1000.0
+    c["traceScale"] = 1.0
+    c["historyCap"] = 1.0
+    return c
+
+
+def _obj(shape, color, pos=(0, 0, 0), radius=0.0, u=(0, 0, 0), v=(0, 0, 0), albedo=(1, 1, 1), fuzz=0.0, ior=1.0,
+         emission=0.0, texture=NO_TEXTURE, tint=DEFAULT_TEXTURE_TINT):
+    o = np.zeros((), OBJECT_DTYPE)
+    o["x"], o["y"], o["z"] = pos
+    o["x2"], o["y2"], o["z2"] = pos
+    o["radius"] = radius
+    o["u_x"], o["u_y"], o["u_z"] = u
+    o["v_x"], o["v_y"], o["v_z"] = v
+    o["r"], o["g"], o["b"] = albedo
+    o["fuzz"], o["refraction"], o["emission"] = fuzz, ior, emission
+    o["shapeType"], o["colorType"], o["textureID"] = shape, color, texture
+    o["textureTint"] = tint
+    return o
+
+
+def sphere_field(seed: int = 0xA11A70E1, width=400, height=225, spp=16, depth=10) -> SceneSpec:
+    """RTIOW final scene (scene_builders.cpp:107-181 for the camera): ground
+    sphere, 22x22 random small spheres, three large ones; analytic spheres,
+    path_trace semantics."""
+    rng = Pcg32(seed)
+    objs = [_obj(SPHERE, LAMBERTIAN, (0, -1000, 0), 1000.0, albedo=(0.5, 0.5, 0.5))]
+    for a in range(-11, 11):
+        for b in range(-11, 11):
+            choose = rng.uniform()
+            center = (a + 0.9 * rng.uniform(), 0.2, b + 0.9 * rng.uniform())
+            if choose < 0.8:
+                c1 = [rng.uniform() for _ in range(3)]
+                c2 = [rng.uniform() for _ in range(3)]
+                objs.append(_obj(SPHERE, LAMBERTIAN, center, 0.2, albedo=tuple(x * y for x, y in zip(c1, c2))))
+            elif choose < 0.95:
+                alb = tuple(rng.uniform(0.5, 1.0) for _ in range(3))
+                objs.append(_obj(SPHERE, METAL, center, 0.2, albedo=alb, fuzz=rng.uniform(0.0, 0.5)))
+            else:
+                objs.append(_obj(SPHERE, DIELECTRIC, center, 0.2, ior=1.5))
+    objs.append(_obj(SPHERE, DIELECTRIC, (0, 1, 0), 1.0, ior=1.5))
+    objs.append(_obj(SPHERE, LAMBERTIAN, (-4, 1, 0), 1.0, albedo=(0.4, 0.2, 0.1)))
+    objs.append(_obj(SPHERE, METAL, (4, 1, 0), 1.0, albedo=(0.7, 0.6, 0.5), fuzz=0.0))
+    cfg = make_config(width, height, spp, depth, (13, 2, 3), (0, 0, 0), 20.0, 10.0, 0.6, (0.5, 0.7, 1.0))
+    return SceneSpec("sphere_field", abi.MODE_PATH_TRACE, np.array(objs, OBJECT_DTYPE), config=cfg, seed=seed)
+
+
+def icosphere_tris(seed: int = 0x1C05F3E6, width=1280, height=720, spp=4, depth=5, level=6) -> SceneSpec:
+    """Level-6 icosphere displaced by 3-octave value noise (amplitude 0.1) as
+    Triangle objects, a two-triangle ground and one emissive quad (radiance
+    15), all in one BVH; ray_trace semantics."""
+    rng = Pcg32(seed)
+    V, F = icosphere(level)
+    disp = 1.0 + 0.1 * fbm3(V, seed & 0xFFFF)
+    P = (V * disp[:, None]).astype(np.float32)
+    alb_sphere = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+    alb_ground = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+    light = _obj(QUAD, DIFFUSE_LIGHT, (-1.0, 3.5, -1.0), u=(2.0, 0.0, 0.0), v=(0.0, 0.0, 2.0), emission=15.0)
+    g = np.array([[-6, -1.15, -6], [6, -1.15, -6], [6, -1.15, 6], [-6, -1.15, 6]], np.float32)
+    tris = np.concatenate([F.astype(np.int64)], 0)
+    n = tris.shape[0] + 2
+    objs = np.zeros(1 + n, OBJECT_DTYPE)
+    objs[0] = light
+    a, b, c = P[tris[:, 0]], P[tris[:, 1]], P[tris[:, 2]]
+    ga = np.stack([g[0], g[0]]), np.stack([g[2], g[3]]), np.stack([g[1], g[2]])
+    A = np.concatenate([a, ga[0]]).astype(np.float32)
+    B = np.concatenate([b, ga[1]]).astype(np.float32)
+    C = np.concatenate([c, ga[2]]).astype(np.float32)
+    U, Vv = (B - A).astype(np.float32), (C - A).astype(np.float32)
+    t = objs[1:]
+    t["x"], t["y"], t["z"] = A[:, 0], A[:, 1], A[:, 2]
+    t["x2"], t["y2"], t["z2"] = A[:, 0], A[:, 1], A[:, 2]
+    t["u_x"], t["u_y"], t["u_z"] = U[:, 0], U[:, 1], U[:, 2]
+    t["v_x"], t["v_y"], t["v_z"] = Vv[:, 0], Vv[:, 1], Vv[:, 2]
+    t["shapeType"] = TRIANGLE
+    t["colorType"] = LAMBERTIAN
+    t["textureID"] = NO_TEXTURE
+    t["textureTint"] = DEFAULT_TEXTURE_TINT
+    t["refraction"] = 1.0
+    t["r"][:-2], t["g"][:-2], t["b"][:-2] = alb_sphere
+    t["r"][-2:], t["g"][-2:], t["b"][-2:] = alb_ground
+    src = (0.0, 1.2, 4.5)
+    focus = math.sqrt(sum(x * x for x in src))
+    cfg = make_config(width, height, spp, depth, src, (0, 0, 0), 40.0, focus, 0.0, (0.02, 0.02, 0.03), exposure=1.0)
+    return SceneSpec("icosphere_tris", abi.MODE_RAY_TRACE, objs, config=cfg, seed=seed)
+
+
+def instanced(seed: int = 0x7E55E11A, width=1920, height=1080, spp=8, depth=8, n_instances=1024, tex_size=4096,
+              proto_scale=1.0, prototypes=None) -> SceneSpec:
+    """Two-level scene: n_instances instances of four prototypes (icosphere
+    L4 5,120 tris; cube grid 12,288 tris; torus 20,000 tris; noise plane
+    257^2 vertices / 131,072 tris), uniform random rotation, scale U[0.5,2],
+    translation U[-100,100]^3, 70/20/10 % Lambertian/metal/dielectric, one
+    Perlin albedo texture, one emissive sphere light."""
+    rng = Pcg32(seed)
+    if prototypes is None:
+        prototypes = [
+            mesh_icosphere(4, 4.0 * proto_scale),
+            mesh_cube_grid(32, 3.5 * proto_scale),
+            mesh_torus(100, 100, 4.0 * proto_scale, 1.4 * proto_scale),
+            mesh_noise_plane(256, 8.0 * proto_scale, 1.5 * proto_scale, seed & 0xFFFF),
+        ]
+    light = _obj(SPHERE, DIFFUSE_LIGHT, (0.0, 220.0, 0.0), 40.0, albedo=(1.0, 0.95, 0.9), emission=6.0)
+    objs = [light]
+    inst = np.zeros(n_instances, INSTANCE_DTYPE)
+    mesh_of = []
+    vel = np.zeros((n_instances, 6), np.float32)
+    for i in range(n_instances):
+        m = i % len(prototypes)
+        mesh_of.append(m)
+        qx, qy, qz, qw = rng.quaternion()
+        s = rng.uniform(0.5, 2.0)
+        p = (rng.uniform(-100, 100), rng.uniform(-100, 100), rng.uniform(-100, 100))
+        inst[i]["px"], inst[i]["py"], inst[i]["pz"] = p
+        inst[i]["scale"] = s
+        inst[i]["qx"], inst[i]["qy"], inst[i]["qz"], inst[i]["qw"] = qx, qy, qz, qw
+        choose = rng.uniform()
+        if choose < 0.7:
+            o = _obj(MESH, LAMBERTIAN, p, albedo=tuple(rng.uniform(0.3, 0.95) for _ in range(3)), texture=0)
+        elif choose < 0.9:
+            o = _obj(MESH, METAL, p, albedo=tuple(rng.uniform(0.5, 1.0) for _ in range(3)), fuzz=rng.uniform(0, 0.3),
+                     texture=0)
+        else:
+            o = _obj(MESH, DIELECTRIC, p, ior=1.5)
+        objs.append(o)
+        vel[i] = [rng.normal() for _ in range(3)] + [0.5 * rng.normal() for _ in range(3)]
+    tex = perlin_texture(tex_size, seed & 0xFFFF)[None] if tex_size else None
+    cfg = make_config(width, height, spp, depth, (0.0, 30.0, 260.0), (0.0, 0.0, 0.0), 45.0, 261.7, 0.0,
+                      (0.35, 0.45, 0.6), exposure=1.0)
+    return SceneSpec("instanced", abi.MODE_RAY_TRACE, np.array(objs, OBJECT_DTYPE), meshes=prototypes,
+                     instance_mesh=mesh_of, instances=inst, texture=tex, config=cfg, seed=seed, velocities=vel)
+
+
+# ---------------------------------------------------------------------------
+# Packing (MeshLibrary + RebuildAcceleration + PackObjects)
+# ---------------------------------------------------------------------------
+class Builder:
+    """BVH builders behind one interface: the product's (libavrt) or another
+    implementation with the same C signatures."""
+
+    def __init__(self, lib=None, kind="avrt"):
+        self.lib = lib if lib is not None else abi.load_library()
+        self.kind = kind
+
+    def build_bvh(self, bounds: np.ndarray):
+        n = len(bounds)
+        nodes = np.zeros(max(2 * n, 1), NODE_DTYPE)
+        order = np.zeros(max(n, 1), np.uint32)
+        bounds = np.ascontiguousarray(bounds, AABB_DTYPE)
+        if self.kind == "avrt":
+            cnt = ctypes.c_uint32(0)
+            rc = self.lib.avrt_build_bvh(abi.ptr(bounds), n, abi.ptr(nodes), ctypes.byref(cnt), abi.ptr(order))
+            if rc:
+                raise abi.AvrtError(rc, "avrt_build_bvh")
+            count = cnt.value
+        else:
+            count = self.lib.orc_build_bvh(abi.ptr(bounds), n, abi.ptr(nodes), abi.ptr(order))
+        return nodes[:count].copy(), order[:n].copy()
+
+    def build_blas(self, pos: np.ndarray, tri: np.ndarray):
+        pos = np.ascontiguousarray(pos, np.float32)
+        tri = np.ascontiguousarray(tri, np.uint32)
+        nt = tri.shape[0]
+        nodes = np.zeros(max(2 * nt, 1), NODE_DTYPE)
+        idx = np.zeros(max(3 * nt, 1), np.uint32)
+        local = np.zeros(1, AABB_DTYPE)
+        if self.kind == "avrt":
+            cnt = ctypes.c_uint32(0)
+            rc = self.lib.avrt_build_blas(abi.ptr(pos), pos.shape[0], abi.ptr(tri), nt, abi.ptr(nodes),
+                                          ctypes.byref(cnt), abi.ptr(idx), abi.ptr(local))
+            if rc:
+                raise abi.AvrtError(rc, "avrt_build_blas")
+            count = cnt.value
+        else:
+            count = self.lib.orc_build_blas(abi.ptr(pos), pos.shape[0], abi.ptr(tri), nt, abi.ptr(nodes),
+                                            abi.ptr(idx), abi.ptr(local))
+        return nodes[:count].copy(), idx[:3 * nt].copy(), local[0].copy()
+
+    def instance_bounds(self, inst: np.ndarray, local: np.ndarray):
+        inst = np.ascontiguousarray(inst, INSTANCE_DTYPE)
+        local = np.ascontiguousarray(local, AABB_DTYPE)
+        out = np.zeros(len(inst), AABB_DTYPE)
+        if len(inst) == 0:
+            return out
+        if self.kind == "avrt":
+            rc = self.lib.avrt_instance_bounds(abi.ptr(inst), len(inst), abi.ptr(local), abi.ptr(out))
+            if rc:
+                raise abi.AvrtError(rc, "avrt_instance_bounds")
+        else:
+            self.lib.orc_instance_bounds(abi.ptr(inst), len(inst), abi.ptr(local), abi.ptr(out))
+        return out
+
+
+@dataclass
+class Packed:
+    """The reference ABI arrays of a scene, ready for avrt_upload_*."""
+    vertices: np.ndarray
+    indices: np.ndarray
+    blas: np.ndarray
+    mesh_ranges: list           # per mesh: (vertexBase, indexBase, triangleCount, blasBase)
+    local_bounds: np.ndarray    # per mesh AABB
+    objects: np.ndarray         # slot order
+    tlas: np.ndarray
+    order: np.ndarray           # slot -> authoring index
+    lights: np.ndarray
+    instances: np.ndarray       # slot order among meshes
+    texture: np.ndarray | None
+    config: np.ndarray
+    bounds: np.ndarray          # authoring-order renderable bounds
+
+
+def mesh_library(meshes: list, builder: Builder):
+    """MeshLibrary::Add for each mesh (scene/mesh_library.h:91-159)."""
+    verts, idxs, blas, ranges, locals_ = [], [], [], [], []
+    vb = ib = bb = 0
+    for m in meshes:
+        nodes, idx, local = builder.build_blas(m.pos, m.tri)
+        v = np.zeros(m.pos.shape[0], VERTEX_DTYPE)
+        v["px"], v["py"], v["pz"] = m.pos[:, 0], m.pos[:, 1], m.pos[:, 2]
+        v["nx"], v["ny"], v["nz"] = m.nrm[:, 0], m.nrm[:, 1], m.nrm[:, 2]
+        v["u"], v["v"] = m.uv[:, 0], m.uv[:, 1]
+        ranges.append((vb, ib, m.tri.shape[0], bb))
+        verts.append(v)
+        idxs.append(idx)
+        blas.append(nodes)
+        locals_.append(local)
+        vb += v.shape[0]
+        ib += idx.shape[0]
+        bb += nodes.shape[0]
+    cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
+    return (cat(verts, VERTEX_DTYPE), cat(idxs, np.uint32), cat(blas, NODE_DTYPE), ranges,
+            np.array(locals_, AABB_DTYPE) if locals_ else np.zeros(0, AABB_DTYPE))
+
+
+def object_bounds(objects: np.ndarray, instances: np.ndarray | None, inst_local: np.ndarray | None,
+                  builder: Builder) -> np.ndarray:
+    """World AABB per renderable: SphereBounds, QuadBounds, TriangleBounds,
+    MeshBounds (scene/shapes.h:71-145), float32 arithmetic in the same order."""
+    n = len(objects)
+    b = np.zeros(n, AABB_DTYPE)
+    f = np.float32
+    P = np.stack([objects["x"], objects["y"], objects["z"]], 1).astype(f)
+    U = np.stack([objects["u_x"], objects["u_y"], objects["u_z"]], 1).astype(f)
+    V = np.stack([objects["v_x"], objects["v_y"], objects["v_z"]], 1).astype(f)
+    R = objects["radius"].astype(f)
+    shape = objects["shapeType"]
+    lo = np.zeros((n, 3), f)
+    hi = np.zeros((n, 3), f)
+    s = shape == SPHERE
+    lo[s] = P[s] - R[s, None]
+    hi[s] = P[s] + R[s, None]
+    q = shape == QUAD
+    if q.any():
+        c0, c1, c2, c3 = P[q], (P[q] + U[q]) + V[q], P[q] + U[q], P[q] + V[q]
+        lo[q] = np.minimum(np.minimum(c0, c1), np.minimum(c2, c3))
+        hi[q] = np.maximum(np.maximum(c0, c1), np.maximum(c2, c3))
+    t = shape == TRIANGLE
+    if t.any():
+        c0, c1, c2 = P[t], P[t] + U[t], P[t] + V[t]
+        lo[t] = np.minimum(np.minimum(c0, c1), c2)
+        hi[t] = np.maximum(np.maximum(c0, c1), c2)
+    m = np.nonzero(shape == MESH)[0]
+    if len(m):
+        wb = builder.instance_bounds(instances, inst_local)
+        for k, i in enumerate(m):
+            lo[i] = (wb[k]["min_x"], wb[k]["min_y"], wb[k]["min_z"])
+            hi[i] = (wb[k]["max_x"], wb[k]["max_y"], wb[k]["max_z"])
+    b["min_x"], b["min_y"], b["min_z"] = lo[:, 0], lo[:, 1], lo[:, 2]
+    b["max_x"], b["max_y"], b["max_z"] = hi[:, 0], hi[:, 1], hi[:, 2]
+    return b
+
+
+def full_instances(spec: SceneSpec, ranges) -> np.ndarray:
+    """Instance records in authoring order with the mesh ranges filled in."""
+    if spec.instances is None:
+        return np.zeros(0, INSTANCE_DTYPE)
+    inst = spec.instances.copy()
+    for k, m in enumerate(spec.instance_mesh):
+        vb, ib, tc, bb = ranges[m]
+        inst[k]["vertexBase"], inst[k]["indexBase"], inst[k]["triangleCount"], inst[k]["blasBase"] = vb, ib, tc, bb
+    return inst
+
+
+def pack(spec: SceneSpec, builder: Builder | None = None, library=None) -> Packed:
+    """Full host packing of a scene: mesh library, TLAS over the renderable
+    bounds, objects/instances/lights in slot order."""
+    builder = builder or Builder()
+    if library is None:
+        library = mesh_library(spec.meshes, builder)
+    verts, idx, blas, ranges, local = library
+    inst_auth = full_instances(spec, ranges)
+    inst_local = np.array([local[m] for m in spec.instance_mesh], AABB_DTYPE) if len(inst_auth) else None
+    bounds = object_bounds(spec.objects, inst_auth, inst_local, builder)
+    return pack_with_bounds(spec, builder, library, inst_auth, bounds)
+
+
+def pack_with_bounds(spec, builder, library, inst_auth, bounds) -> Packed:
+    verts, idx, blas, ranges, local = library
+    tlas, order = builder.build_bvh(bounds) if len(bounds) else (np.zeros(0, NODE_DTYPE), np.zeros(0, np.uint32))
+    objects = spec.objects[order].copy()
+    mesh_rank = np.full(len(spec.objects), -1, np.int64)
+    mesh_rank[np.nonzero(spec.objects["shapeType"] == MESH)[0]] = np.arange(int((spec.objects["shapeType"] == MESH).sum()))
+    inst_slots = []
+    for slot, a in enumerate(order):
+        if objects[slot]["shapeType"] == MESH:
+            objects[slot]["instanceIndex"] = len(inst_slots)
+            inst_slots.append(inst_auth[mesh_rank[a]])
+    instances = np.array(inst_slots, INSTANCE_DTYPE) if inst_slots else np.zeros(0, INSTANCE_DTYPE)
+    lights = np.nonzero(objects["colorType"] == DIFFUSE_LIGHT)[0].astype(np.uint32)
+    cfg = spec.config.copy()
+    cfg["num_spheres"] = len(objects)
+    cfg["num_lights"] = len(lights)
+    return Packed(verts, idx, blas, ranges, local, objects, tlas, order, lights, instances, spec.texture, cfg, bounds)
+
+
+def write_color_gamma2(image_rgba32f: np.ndarray) -> np.ndarray:
+    """core/color.h:32-50: gamma 2 (sqrt of positive values), then
+    int(256 * clamp(c, 0, 0.999)) per channel. Returns uint8 (h, w, 3)."""
+    c = image_rgba32f[..., :3].astype(np.float64)
+    g = np.where(c > 0, np.sqrt(np.maximum(c, 0)), 0.0)
+    return (256.0 * np.clip(g, 0.0, 0.999)).astype(np.uint8)

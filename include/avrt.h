@@ -1,0 +1,344 @@
+/*
+ * avrt.h - C ABI of the MI355X ray-tracing library (libavrt.so).
+ *
+ * This is the drop-in boundary for the ray-tracing path of AviatorEngine2.
+ * The reference drives its shaders from Renderer (renderer/renderer.h:325-496)
+ * through SDL_gpu: it uploads geometry once, uploads the object/BVH/light/
+ * instance arrays whenever the scene is dirty, pushes a Config cbuffer and
+ * dispatches a compute shader. Every entry point below replaces one of those
+ * steps; the structs keep the reference's byte layouts so the arrays a
+ * Renderer already builds can be handed over unchanged.
+ *
+ * Conventions
+ *   - Every function returns an int status: AVRT_OK (0) or a negative AVRT_E_*
+ *     code. Nothing throws across this boundary. avrt_last_error() returns a
+ *     human-readable reason for the last failure on a context.
+ *   - Host arrays passed in are borrowed for the duration of the call only
+ *     (copied to device memory before returning).
+ *   - A context owns all of its device memory and one HIP stream. Calls on one
+ *     context must be serialised by the caller (the reference is single
+ *     threaded too, renderer.cpp:1332); different contexts may be used from
+ *     different threads.
+ */
+#ifndef AVRT_H
+#define AVRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes (renderer.cpp returns bool + SDL_Log; this ABI returns codes). */
+/* ------------------------------------------------------------------------- */
+#define AVRT_OK 0
+#define AVRT_E_INVALID_ARG (-1)   /* null pointer, zero size, bad enum            */
+#define AVRT_E_CAPACITY (-2)      /* upload larger than the capacity fixed at load */
+#define AVRT_E_OOM (-3)           /* device allocation failed                      */
+#define AVRT_E_DEVICE (-4)        /* a HIP call failed                             */
+#define AVRT_E_STATE (-5)         /* call order violated (e.g. render before load) */
+#define AVRT_E_BUDGET (-6)        /* serving: job does not fit the HBM budget      */
+#define AVRT_E_UNSUPPORTED (-7)   /* feature outside this build                    */
+
+/* Material and shape enums, values as uploaded by the reference.
+ * MaterialType: scene/material.h:11-18. BodyShape: scene/gpu_types.h:14-21. */
+#define AVRT_LAMBERTIAN 0u
+#define AVRT_METAL 1u
+#define AVRT_DIELECTRIC 2u
+#define AVRT_DIFFUSE_LIGHT 3u
+#define AVRT_ISOTROPIC 4u
+
+#define AVRT_SHAPE_SPHERE 0u
+#define AVRT_SHAPE_QUAD 1u
+#define AVRT_SHAPE_BOX 2u
+#define AVRT_SHAPE_TRIANGLE 3u
+#define AVRT_SHAPE_MESH 4u
+
+#define AVRT_NO_TEXTURE 0xFFFFFFFFu /* kNoTexture, scene/texture.h */
+
+/* ------------------------------------------------------------------------- */
+/* Byte layouts (the reference ABI, with the same offsets).                  */
+/* ------------------------------------------------------------------------- */
+
+/* Object_GPU, scene/gpu_types.h:26-81 (128 bytes). */
+typedef struct AvrtObject {
+    float x, y, z;          /*   0 position                                  */
+    float prevRotX;         /*  12                                           */
+    float x2, y2, z2;       /*  16 previous position (motion)                */
+    float radius;           /*  28                                           */
+    float u_x, u_y, u_z;    /*  32 quad / triangle edge u                    */
+    float prevRotY;         /*  44                                           */
+    float v_x, v_y, v_z;    /*  48 quad / triangle edge v                    */
+    float prevRotZ;         /*  60                                           */
+    float half_x, half_y, half_z; /* 64 box half extents                     */
+    float rotation;         /*  76 yaw (sphere uv spin / box yaw)            */
+    float density;          /*  80                                           */
+    float emission;         /*  84                                           */
+    uint32_t instanceIndex; /*  88                                           */
+    float textureTint;      /*  92                                           */
+    float r, g, b;          /*  96 albedo                                    */
+    float fuzz;             /* 108                                           */
+    float refraction;       /* 112 ior                                       */
+    uint32_t shapeType;     /* 116 AVRT_SHAPE_*                              */
+    uint32_t colorType;     /* 120 AVRT_LAMBERTIAN..                         */
+    uint32_t textureID;     /* 124 texture layer or AVRT_NO_TEXTURE          */
+} AvrtObject;
+
+/* BVHNode_GPU, scene/bvh.h:31-41 (48 bytes). A leaf has count > 0 and `left`
+ * is its first primitive slot; an interior node has count == 0 and left/right
+ * are node indices. Node 0 is the root. */
+typedef struct AvrtBVHNode {
+    float min_x, min_y, min_z;
+    float pad0;
+    float max_x, max_y, max_z;
+    int32_t left;
+    int32_t right;
+    int32_t count;
+    int32_t padding;
+    int32_t pad_trailing;
+} AvrtBVHNode;
+
+/* MeshInstance_GPU, scene/gpu_types.h:84-95 (48 bytes).
+ * ABI extension: `scale` occupies the reference's pad0 slot (offset 12) and is
+ * a uniform scale; 0 (what the reference writes there) means 1. */
+typedef struct AvrtMeshInstance {
+    float px, py, pz;
+    float scale;
+    float qx, qy, qz, qw;
+    uint32_t vertexBase;
+    uint32_t indexBase;
+    uint32_t triangleCount;
+    uint32_t blasBase;
+} AvrtMeshInstance;
+
+/* Vertex_GPU, scene/mesh_library.h:39-44 (32 bytes). */
+typedef struct AvrtVertex {
+    float px, py, pz;
+    float nx, ny, nz;
+    float u, v;
+} AvrtVertex;
+
+/* AABB, scene/aabb.h:14-18 (24 bytes). */
+typedef struct AvrtAABB {
+    float min_x, min_y, min_z;
+    float max_x, max_y, max_z;
+} AvrtAABB;
+
+/* Config cbuffer, renderer/renderer.h:76-148 (240 bytes). */
+typedef struct AvrtConfig {
+    float source_x, source_y, source_z;  /*   0 */
+    float fov;                           /*  12 */
+    float target_x, target_y, target_z;  /*  16 */
+    float focus_dist;                    /*  28 */
+    float defocus_angle;                 /*  32 */
+    float padding[3];                    /*  36 */
+    float up_x, up_y, up_z;              /*  48 */
+    float oof;                           /*  60 */
+    float sky_r, sky_g, sky_b;           /*  64 */
+    uint32_t width;                      /*  76 */
+    float horizon_r, horizon_g, horizon_b; /* 80 */
+    uint32_t height;                     /*  92 */
+    uint32_t samples;                    /*  96 */
+    uint32_t batches;                    /* 100 */
+    uint32_t batch;                      /* 104 */
+    uint32_t depth;                      /* 108 */
+    uint32_t num_spheres;                /* 112 */
+    uint32_t renderType;                 /* 116 */
+    uint32_t num_lights;                 /* 120 */
+    float exposure;                      /* 124 */
+    uint32_t debugView;                  /* 128 */
+    float zNear;                         /* 132 */
+    float zFar;                          /* 136 */
+    float jitter_x;                      /* 140 */
+    float jitter_y;                      /* 144 */
+    uint32_t displayOnly;                /* 148 */
+    float traceScale;                    /* 152 */
+    uint32_t frameIndex;                 /* 156 */
+    uint32_t historyReset;               /* 160 */
+    uint32_t cameraMoved;                /* 164 */
+    float historyCap;                    /* 168 */
+    float padding3;                      /* 172 */
+    float prevViewProj[16];              /* 176 row-major 4x4 */
+} AvrtConfig;
+
+/* ------------------------------------------------------------------------- */
+/* Render modes.                                                             */
+/* ------------------------------------------------------------------------- */
+/* shaders/path_trace.comp.hlsl:781-860: RTIOW integrator, sky gradient,
+ * emission + BSDF bounce, no NEE, no Russian roulette, rgba32f average. */
+#define AVRT_MODE_PATH_TRACE 0
+/* shaders/ray_trace.comp.hybrid.hlsl:1328-1404: compute-only renderer with
+ * two-level TLAS->BLAS, NEE with shadow rays through glass, Russian roulette,
+ * flat sky, running-average accumulation and ACES + sRGB display. */
+#define AVRT_MODE_RAY_TRACE 1
+
+/* Sample semantics: a call with config.samples = S and config.batch = B
+ * renders S independent reference frames at Samples = 1 with Batch = B .. B+S-1
+ * (seed = x + y*W + Batch*W*H + 1 per frame, ray_trace.comp.hybrid.hlsl:1346)
+ * and folds them into the accumulation image in that order, exactly as S
+ * consecutive RenderFrame calls would (renderer.cpp:1349). */
+
+/* Context creation options. */
+typedef struct AvrtCreateOptions {
+    int device;             /* HIP device ordinal                              */
+    uint32_t max_width;     /* largest image this context will render          */
+    uint32_t max_height;
+    uint32_t max_paths;     /* path slots per wave; 0 = width*height*samples    */
+    uint32_t fp16_accum;    /* 1: round the accumulation image through FP16
+                               after every frame, like the reference's RGBA16F
+                               target (renderer.cpp:251)                       */
+    uint32_t traversal_order; /* 0: nearer child first (fastest). 1: the
+                               reference's order, right child first
+                               (ray_trace.comp.hybrid.hlsl:909-913), so hits
+                               at exactly equal t resolve as in the reference */
+    uint32_t reserved[2];
+} AvrtCreateOptions;
+
+/* Counters of the last avrt_render call. */
+typedef struct AvrtStats {
+    uint64_t rays_primary;   /* camera rays traced                    */
+    uint64_t rays_bounce;    /* bounce (extension) rays traced         */
+    uint64_t rays_shadow;    /* shadow queries (incl. glass steps)     */
+    uint64_t paths;          /* camera paths (samples)                 */
+    float ms_total;          /* device time of the whole call          */
+    float ms_extend;         /* device time in closest-hit traversal   */
+    float ms_shade;          /* device time in shading                 */
+    float ms_shadow;         /* device time in shadow traversal        */
+    float ms_other;          /* raygen, resolve, TLAS rebuild          */
+    uint32_t launches_extend;/* number of extend launches              */
+    uint32_t reserved[3];
+} AvrtStats;
+
+typedef struct avrt_ctx avrt_ctx;
+
+/* Renderer::Initialize (renderer/renderer.cpp:104-215): device, stream, and
+ * the accumulation/display targets. */
+int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx);
+
+/* Renderer::Shutdown (renderer/renderer.h:418). */
+int avrt_destroy(avrt_ctx *ctx);
+
+/* Text of the last failure on ctx (or of a failed avrt_create when ctx is
+ * null). Never null. */
+const char *avrt_last_error(const avrt_ctx *ctx);
+
+/* Renderer::UploadMeshGeometry (renderer/renderer.cpp:706-785): the mesh
+ * library's concatenated local-space vertices, BLAS-ordered triangle indices
+ * and BLAS nodes (mesh-relative indices, scene/mesh_library.h:134-155).
+ * ABI extension: indices are 32-bit (the reference Mesh uses uint16_t,
+ * scene/mesh.h:81). */
+int avrt_upload_geometry(avrt_ctx *ctx,
+                         const AvrtVertex *vertices, uint32_t vertex_count,
+                         const uint32_t *indices, uint32_t index_count,
+                         const AvrtBVHNode *blas, uint32_t blas_count);
+
+/* Renderer::UploadScene (renderer/renderer.cpp:931-1027): objects in TLAS
+ * slot order, TLAS nodes, light slots and mesh instances. The bind order of
+ * the reference (objects, bvh, lightIDs, vertices, meshIndex, blas, instances;
+ * renderer.cpp:1456-1459) is the argument order here and in
+ * avrt_upload_geometry. An empty scene (object_count == 0) is allowed and
+ * renders sky only. */
+int avrt_upload_scene(avrt_ctx *ctx,
+                      const AvrtObject *objects, uint32_t object_count,
+                      const AvrtBVHNode *tlas, uint32_t node_count,
+                      const uint32_t *light_ids, uint32_t light_count,
+                      const AvrtMeshInstance *instances, uint32_t instance_count);
+
+/* TextureLibrary::BuildGPUArray (scene/texture.h:110-214): `layers` RGBA8
+ * layers of width*height texels, sRGB encoded. ABI extension: layers up to
+ * 8192^2 (the reference caps at 2048^2, texture.h:22). Sampled bilinear,
+ * REPEAT, level 0, sRGB decoded to linear before filtering
+ * (renderer.cpp:226-233, ray_trace.comp.hybrid.hlsl:919-926). */
+int avrt_upload_textures(avrt_ctx *ctx, const uint8_t *rgba8_srgb,
+                         uint32_t width, uint32_t height, uint32_t layers);
+
+/* Renderer::RenderFrame compute-only dispatch (renderer.cpp:1426-1494).
+ * mode is AVRT_MODE_PATH_TRACE or AVRT_MODE_RAY_TRACE. Blocks until the frame
+ * is finished. */
+int avrt_render(avrt_ctx *ctx, const AvrtConfig *config, int mode);
+
+/* Counters and device times of the last render. */
+int avrt_get_stats(const avrt_ctx *ctx, AvrtStats *out);
+
+/* Read back the accumulation image as RGBA32F (width*height*16 bytes) and the
+ * display image as RGBA8 (width*height*4 bytes; hybrid mode only). */
+int avrt_read_image(avrt_ctx *ctx, float *dst, size_t bytes);
+int avrt_read_display(avrt_ctx *ctx, uint8_t *dst, size_t bytes);
+
+/* Primary visibility only: for sample `config.batch` of every pixel, the TLAS
+ * slot of the closest object (0xFFFFFFFF on a miss), the mesh triangle slot
+ * (BLAS slot, 0xFFFFFFFF when not a mesh) and the hit distance t. The rays are
+ * the camera rays of the first sample of ray_trace.comp.hybrid.hlsl:1368-1373
+ * (path_trace.comp.hlsl:806-811 is identical). Any pointer may be null. */
+int avrt_trace_primary(avrt_ctx *ctx, const AvrtConfig *config,
+                       uint32_t *hit_object, uint32_t *hit_prim, float *hit_t);
+
+/* ------------------------------------------------------------------------- */
+/* Acceleration structures.                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* BuildBVH (scene/bvh.h:275-295) on the host: 12-bin SAH, leaves of <= 4,
+ * depth cap 20, flat-axis inflation, DFS preorder numbering. out_nodes must
+ * hold 2*n nodes and out_order n entries; *out_node_count receives the number
+ * of nodes written. Bit-identical to the reference on libstdc++. */
+int avrt_build_bvh(const AvrtAABB *bounds, uint32_t n,
+                   AvrtBVHNode *out_nodes, uint32_t *out_node_count,
+                   uint32_t *out_order);
+
+/* MeshLibrary::Add (scene/mesh_library.h:91-159) for one mesh: triangle
+ * AABBs, BuildBVH, and the index buffer reordered by the BLAS permutation.
+ * positions are xyz per vertex; tri_indices are 3 per triangle. out_nodes must
+ * hold 2*tri_count nodes, out_indices 3*tri_count entries, out_local_bounds
+ * one AABB (the vertex bounds). */
+int avrt_build_blas(const float *positions, uint32_t vertex_count,
+                    const uint32_t *tri_indices, uint32_t tri_count,
+                    AvrtBVHNode *out_nodes, uint32_t *out_node_count,
+                    uint32_t *out_indices, AvrtAABB *out_local_bounds);
+
+/* MeshBounds (scene/shapes.h:124-145): world AABB of each instance from its
+ * mesh's local bounds (8 rotated corners; scale applied first). Host code. */
+int avrt_instance_bounds(const AvrtMeshInstance *instances, uint32_t n,
+                         const AvrtAABB *local_bounds_per_instance,
+                         AvrtAABB *out_bounds);
+
+/* RebuildAcceleration (renderer.cpp:1087-1130) on the device for a scene of
+ * mesh instances plus analytic objects: `objects_in`/`instances_in` are in
+ * authoring order (instances_in[k] belongs to the k-th mesh object), the
+ * world AABBs are computed on the device (MeshBounds / SphereBounds /
+ * QuadBounds / TriangleBounds), the TLAS is built with the same binned SAH as
+ * avrt_build_bvh (bit-identical arrays), and objects, instances and light
+ * slots are written in slot order into the context's scene buffers, as
+ * PackObjects + LightIndices do (renderer.cpp:803-929). local_bounds holds
+ * one AABB per instance. Requires a previous avrt_upload_geometry. */
+int avrt_scene_set_authoring(avrt_ctx *ctx,
+                             const AvrtObject *objects_in, uint32_t object_count,
+                             const AvrtMeshInstance *instances_in, uint32_t instance_count,
+                             const AvrtAABB *local_bounds);
+/* Advance the rigid bodies by dt (positions += v*dt, orientation integrated
+ * by the angular velocity) and rebuild the TLAS on the device. velocities is
+ * 6 floats per instance (linear xyz, angular xyz); pass null to rebuild only. */
+int avrt_scene_step(avrt_ctx *ctx, const float *velocities, float dt);
+/* Copy the device-built TLAS and slot-ordered objects/instances back (for
+ * parity tests). Any pointer may be null. */
+int avrt_scene_read(avrt_ctx *ctx, AvrtBVHNode *nodes, uint32_t *node_count,
+                    AvrtObject *objects, AvrtMeshInstance *instances,
+                    uint32_t *order);
+
+/* BLAS refit on the device after vertex positions changed: recomputes every
+ * BLAS node box bottom-up from the current vertices (same inflation rule as
+ * the builder, scene/bvh.h:58-89). Topology is unchanged. */
+int avrt_refit_blas(avrt_ctx *ctx);
+int avrt_update_vertices(avrt_ctx *ctx, const AvrtVertex *vertices,
+                         uint32_t first, uint32_t count);
+int avrt_read_blas(avrt_ctx *ctx, AvrtBVHNode *dst, uint32_t count);
+
+/* Library version string. */
+const char *avrt_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* AVRT_H */

@@ -1,0 +1,123 @@
+"""CPU oracle for the ray-tracing path (test infrastructure only).
+
+liboracle.so is a strict-FP32 C++ restatement of the reference shaders and
+BVH builder (see oracle.cpp for the file:line map). Only tests/, the smoke
+check and the CPU-baseline leg of bench.py use it, and only as the checker or
+as the CPU baseline - never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SIG = {
+    "orc_build_bvh": (ctypes.c_int, [_P, _U32, _P, _P]),
+    "orc_build_blas": (ctypes.c_int, [_P, _U32, _P, _U32, _P, _P, _P]),
+    "orc_instance_bounds": (None, [_P, _U32, _P, _P]),
+    "orc_sincos": (None, [ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "orc_random": (ctypes.c_float, [ctypes.POINTER(_U32)]),
+    "orc_round_half": (ctypes.c_float, [ctypes.c_float]),
+    "orc_srgb_lut": (None, [_P]),
+    "orc_camera": (None, [_P, _P]),
+    "orc_trace_primary": (None, [_P, _P, _U32, _P, _U32, _P, _P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
+    "orc_render": (ctypes.c_double, [_P, ctypes.c_int, _P, _U32, _P, _U32, _P, _U32, _P, _P, _P, _P, _P, _U32, _U32,
+                                     _U32, _U32, _U32, ctypes.c_int, ctypes.c_int, _P, _P, _P, _U32]),
+    "orc_closest_fp64": (None, [_P, _U32, _P, _U32, _P, _P, _P, _P, _P, _P]),
+    "orc_sizeof_config": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIG.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _geometry(packed):
+    tex = packed.texture
+    if tex is None:
+        return None, 0, 0, 0
+    t = np.ascontiguousarray(tex, np.uint8)
+    return t, t.shape[2], t.shape[1], t.shape[0]
+
+
+def render(packed, config, mode, row0=0, row1=None, threads=None, fp16=False, image=None, stats=False,
+           pixel_stride=0):
+    """Render `config.samples` frames starting at `config.batch` into an
+    RGBA32F image (and RGBA8 display in ray-trace mode). pixel_stride > 1
+    renders only every pixel_stride-th pixel. Returns (image, display,
+    seconds, counters) with counters = {rays, nodes, objects, instances,
+    triangles} of extension rays followed by the same for shadow rays."""
+    lib = load()
+    cfg = np.array(config, copy=True).reshape(1)
+    W, H = int(cfg["width"][0]), int(cfg["height"][0])
+    row1 = H if row1 is None else row1
+    threads = threads or os.cpu_count() or 1
+    img = np.zeros((H, W, 4), np.float32) if image is None else np.ascontiguousarray(image, np.float32)
+    disp = np.zeros((H, W, 4), np.uint8)
+    cnt = np.zeros(10, np.uint64)
+    t, tw, th, tl = _geometry(packed)
+    secs = lib.orc_render(_p(cfg), int(mode), _p(packed.objects), len(packed.objects), _p(packed.tlas), len(packed.tlas),
+                          _p(packed.lights), len(packed.lights), _p(packed.vertices), _p(packed.indices),
+                          _p(packed.blas), _p(packed.instances), _p(t), tw, th, tl, row0, row1, int(threads),
+                          1 if fp16 else 0, _p(img), _p(disp), _p(cnt) if stats else None, int(pixel_stride))
+    return img, disp, secs, cnt
+
+
+def trace_primary(packed, config, p0=0, p1=None, with_rays=False):
+    lib = load()
+    cfg = np.array(config, copy=True).reshape(1)
+    P = int(cfg["width"][0]) * int(cfg["height"][0])
+    p1 = P if p1 is None else p1
+    n = p1 - p0
+    obj = np.empty(n, np.uint32)
+    prim = np.empty(n, np.uint32)
+    t = np.empty(n, np.float32)
+    rays = np.empty((n, 6), np.float32) if with_rays else None
+    lib.orc_trace_primary(_p(cfg), _p(packed.objects), len(packed.objects), _p(packed.tlas), len(packed.tlas),
+                          _p(packed.vertices), _p(packed.indices), _p(packed.blas), _p(packed.instances), p0, p1,
+                          _p(obj), _p(prim), _p(t), _p(rays))
+    return obj, prim, t, rays
+
+
+def closest_fp64(packed, rays):
+    lib = load()
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[0]
+    obj = np.empty(n, np.uint32)
+    prim = np.empty(n, np.uint32)
+    t = np.empty(n, np.float64)
+    lib.orc_closest_fp64(_p(rays), n, _p(packed.objects), len(packed.objects), _p(packed.vertices),
+                         _p(packed.indices), _p(packed.instances), _p(obj), _p(prim), _p(t))
+    return obj, prim, t

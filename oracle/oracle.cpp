@@ -1,0 +1,1352 @@
+// oracle.cpp - CPU restatement of the AviatorEngine2 ray-tracing path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker that the HIP library is
+// compared against; it is never linked into, loaded by or called from the
+// product library. Only tests/, the smoke check and the CPU-baseline leg of
+// bench.py load it.
+//
+// It restates, in strict FP32 (built with -ffp-contract=off, no fast-math),
+// one scalar invocation per pixel exactly like the reference shaders:
+//   BuildBVH / BuildRecursive / MakeNode / SahBin / HalfArea  scene/bvh.h:58-295
+//   Surround / SurroundPoint / Surround2Points               scene/aabb.h:70-103
+//   MeshLibrary::Add (triangle AABBs, index reorder)          scene/mesh_library.h:91-159
+//   MeshBounds (8 rotated corners)                            scene/shapes.h:124-145
+//   rotate(q, v)                                              math/quat.hpp:212-217
+//   Random / RandomDirection                                  shaders/ray_trace.comp.hybrid.hlsl:238-251
+//   HitSphere/SphereSpan/HitQuad/HitTriangle/HitBox           ray_trace.comp.hybrid.hlsl:285-513,729-773
+//   ToLocal/quat_rotate/IntersectAABB/HitMeshTriangle/HitMesh ray_trace.comp.hybrid.hlsl:518-707
+//   HitObject / TraverseBVH                                   ray_trace.comp.hybrid.hlsl:827-917
+//   SurfaceAlbedo / GetBounce / SampleLight / CalculateDirectLight / ColorRay
+//                                                             ray_trace.comp.hybrid.hlsl:940-1325
+//   main (camera, accumulation, ACES, sRGB)                   ray_trace.comp.hybrid.hlsl:1328-1404
+//   path_trace main / GetBounce / SurfaceEmission             shaders/path_trace.comp.hlsl:675-860
+//
+// Choices where HLSL leaves the result open (each is shared with the HIP
+// path and stated here so the oracle stands alone):
+//   * normalize(v) = v * (1 / sqrt(dot(v, v))), dot = (x*x' + y*y') + z*z'.
+//   * sin/cos of RandomDirection use rt_sincos below: Cody-Waite reduction by
+//     pi/2 in three parts and Taylor polynomials of degree 9 (sin) and 10 (cos)
+//     on [-pi/4, pi/4]; tests pin it to libm within 2 ulp.
+//   * pow(x, 2) and pow(x, 5) in Schlick's approximation are evaluated as
+//     x*x and ((x*x)*(x*x))*x.
+//   * min/max follow IEEE minNum/maxNum (a NaN operand yields the other one).
+//   * The camera basis (tan of the field of view) is computed on the host.
+//   * Texture sampling (hardware SampleLevel in the reference) is bilinear,
+//     REPEAT, level 0, with each texel decoded sRGB->linear through a
+//     256-entry table computed in double; this is the build's own sampler
+//     definition (the reference's Vulkan sampler cannot be run here).
+//   * Volumes (ISOTROPIC) are not restated: they draw random numbers inside
+//     traversal, so their streams depend on traversal order.
+//
+// All entry points are extern "C" with plain pointers, for ctypes.
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Reference byte layouts (restated; tests check sizes and offsets).
+// ---------------------------------------------------------------------------
+struct Object {
+    float x, y, z, prevRotX, x2, y2, z2, radius;
+    float u_x, u_y, u_z, prevRotY, v_x, v_y, v_z, prevRotZ;
+    float half_x, half_y, half_z, rotation, density, emission;
+    uint32_t instanceIndex;
+    float textureTint, r, g, b, fuzz, refraction;
+    uint32_t shapeType, colorType, textureID;
+};
+static_assert(sizeof(Object) == 128, "Object layout");
+
+struct Node {
+    float min_x, min_y, min_z, pad0, max_x, max_y, max_z;
+    int32_t left, right, count, padding, pad_trailing;
+};
+static_assert(sizeof(Node) == 48, "Node layout");
+
+struct Inst {
+    float px, py, pz, scale, qx, qy, qz, qw;
+    uint32_t vertexBase, indexBase, triangleCount, blasBase;
+};
+static_assert(sizeof(Inst) == 48, "Inst layout");
+
+struct Vertex {
+    float px, py, pz, nx, ny, nz, u, v;
+};
+static_assert(sizeof(Vertex) == 32, "Vertex layout");
+
+struct AABB {
+    float min_x, min_y, min_z, max_x, max_y, max_z;
+};
+
+struct Config {
+    float source[3], fov, target[3], focus, defocus, pad[3], up[3], oof;
+    float sky[3];
+    uint32_t width;
+    float horizon[3];
+    uint32_t height, samples, batches, batch, depth, num_spheres, renderType, num_lights;
+    float exposure;
+    uint32_t debugView;
+    float zNear, zFar, jitter_x, jitter_y;
+    uint32_t displayOnly;
+    float traceScale;
+    uint32_t frameIndex, historyReset, cameraMoved;
+    float historyCap, padding3;
+    float prevViewProj[16];
+};
+static_assert(sizeof(Config) == 240, "Config layout");
+
+enum : uint32_t { LAMBERTIAN = 0, METAL = 1, DIELECTRIC = 2, DIFFUSE_LIGHT = 3, ISOTROPIC = 4 };
+enum : uint32_t { SPHERE = 0, QUAD = 1, BOX = 2, TRIANGLE = 3, MESH = 4 };
+constexpr uint32_t NO_TEXTURE = 0xFFFFFFFFu;
+constexpr float TAU = 6.2831853f;
+constexpr float PI = 3.14159265f;
+
+// ---------------------------------------------------------------------------
+// FP32 vector helpers (HLSL semantics, evaluated left to right).
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline V3 operator*(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+static inline V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline V3 operator/(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b)
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+static inline V3 normalize(V3 a)
+{
+    const float inv = 1.0f / std::sqrt(dot(a, a));
+    return a * inv;
+}
+static inline V3 lerp(V3 a, V3 b, float t) { return a + t * (b - a); }
+static inline float fminn(float a, float b) { return std::fmin(a, b); }
+static inline float fmaxn(float a, float b) { return std::fmax(a, b); }
+static inline V3 vmin(V3 a, V3 b) { return v3(fminn(a.x, b.x), fminn(a.y, b.y), fminn(a.z, b.z)); }
+static inline V3 vmax(V3 a, V3 b) { return v3(fmaxn(a.x, b.x), fmaxn(a.y, b.y), fmaxn(a.z, b.z)); }
+static inline float saturate(float x) { return fminn(fmaxn(x, 0.0f), 1.0f); }
+static inline V3 reflect(V3 i, V3 n) { return i - 2.0f * n * dot(i, n); }
+static inline V3 refract(V3 i, V3 n, float eta)
+{
+    const float d = dot(n, i);
+    const float k = 1.0f - eta * eta * (1.0f - d * d);
+    if (k < 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    return eta * i - (eta * d + std::sqrt(k)) * n;
+}
+static inline float sgn(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// Sine and cosine: Cody-Waite reduction by pi/2, Taylor polynomials.
+static inline void rt_sincos(float a, float *s, float *c)
+{
+    const float k = std::nearbyint(a * 0.636619772f);
+    float x = a - k * 1.5703125f;
+    x = x - k * 4.837512969970703125e-4f;
+    x = x - k * 7.54978995489188216e-8f;
+    const float x2 = x * x;
+    float ps = 2.75573192e-6f;
+    ps = ps * x2 - 1.98412698e-4f;
+    ps = ps * x2 + 8.33333333e-3f;
+    ps = ps * x2 - 1.66666667e-1f;
+    const float sn = x + (x * x2) * ps;
+    float pc = -2.75573192e-7f;
+    pc = pc * x2 + 2.48015873e-5f;
+    pc = pc * x2 - 1.38888889e-3f;
+    pc = pc * x2 + 4.16666667e-2f;
+    pc = pc * x2 - 0.5f;
+    const float cs = 1.0f + x2 * pc;
+    const int q = static_cast<int>(k) & 3;
+    switch (q) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// RNG (ray_trace.comp.hybrid.hlsl:238-251)
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint32_t seed;
+    float next()
+    {
+        seed = seed * 747796405u + 2891336453u;
+        const uint32_t x = ((seed >> ((seed >> 28u) + 4u)) ^ seed) * 277803737u;
+        return static_cast<float>((x >> 22u) ^ x) / 4294967296.0f;
+    }
+    V3 direction()
+    {
+        const float z = next() * 2.0f - 1.0f;
+        const float a = next() * TAU;
+        const float r = std::sqrt(1.0f - z * z);
+        float s, c;
+        rt_sincos(a, &s, &c);
+        return v3(r * c, r * s, z);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Scene view
+// ---------------------------------------------------------------------------
+struct Scene {
+    const Object *objects;
+    uint32_t nobj;
+    const Node *bvh;
+    uint32_t nnode;
+    const uint32_t *lights;
+    uint32_t nlights;
+    const Vertex *verts;
+    const uint32_t *idx;
+    const Node *blas;
+    const Inst *inst;
+    const uint8_t *tex;
+    uint32_t tw, th, tl;
+};
+
+struct Counters {
+    uint64_t rays = 0, nodes = 0, objs = 0, insts = 0, tris = 0;
+};
+
+struct Ray {
+    V3 o, d;
+    float time;
+};
+
+struct Hit {
+    float t;
+    V3 pos, normal;
+    float u, v;
+    bool face;
+    uint32_t obj;
+    uint32_t prim; // mesh triangle slot (index base), ~0u otherwise
+};
+
+static inline V3 pos_of(const Object &o) { return v3(o.x, o.y, o.z); }
+static inline V3 pos2_of(const Object &o) { return v3(o.x2, o.y2, o.z2); }
+static inline V3 uvec(const Object &o) { return v3(o.u_x, o.u_y, o.u_z); }
+static inline V3 vvec(const Object &o) { return v3(o.v_x, o.v_y, o.v_z); }
+static inline V3 albedo_of(const Object &o) { return v3(o.r, o.g, o.b); }
+
+static inline V3 rotate_y(V3 v, float s, float c) { return v3(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
+
+static void sphere_uv(V3 p, float rotation, float *u, float *v)
+{
+    const float theta = std::acos(-p.y);
+    const float phi = std::atan2(-p.z, p.x) + PI;
+    const float a = phi / TAU - rotation / TAU;
+    *u = a - std::floor(a);
+    *v = theta / PI;
+}
+
+static bool sphere_span(const Object &o, const Ray &r, float *t0, float *t1)
+{
+    const V3 centre = lerp(pos_of(o), pos2_of(o), r.time);
+    const V3 off = centre - r.o;
+    const float a = dot(r.d, r.d);
+    const float b = dot(r.d, off);
+    const float c = dot(off, off) - o.radius * o.radius;
+    const float disc = b * b - a * c;
+    if (disc < 0.0f) return false;
+    const float sq = std::sqrt(disc);
+    *t0 = (b - sq) / a;
+    *t1 = (b + sq) / a;
+    return true;
+}
+
+static bool hit_sphere(const Object &o, uint32_t id, const Ray &r, float near, float far, Hit &h)
+{
+    float t0, t1;
+    if (!sphere_span(o, r, &t0, &t1)) return false;
+    float root = t0;
+    if (root <= near || far <= root) {
+        root = t1;
+        if (root <= near || far <= root) return false;
+    }
+    const V3 centre = lerp(pos_of(o), pos2_of(o), r.time);
+    h.t = root;
+    h.pos = r.o + r.d * root;
+    h.normal = (h.pos - centre) / o.radius;
+    h.face = dot(r.d, h.normal) < 0.0f;
+    if (!h.face) h.normal = -h.normal;
+    if (o.textureID != NO_TEXTURE) {
+        sphere_uv(h.normal, o.rotation, &h.u, &h.v);
+    } else {
+        h.u = h.v = 0.0f;
+    }
+    h.obj = id;
+    h.prim = ~0u;
+    return true;
+}
+
+static bool hit_planar(const Object &o, uint32_t id, const Ray &r, float near, float far, bool tri, Hit &h)
+{
+    const V3 n = cross(uvec(o), vvec(o));
+    const float nl = dot(n, n);
+    if (nl < 1e-8f) return false;
+    const V3 unit = n / std::sqrt(nl);
+    const V3 w = n / nl;
+    const float denom = dot(n, r.d);
+    if (std::fabs(denom) < 1e-8f) return false;
+    const float t = (dot(n, pos_of(o)) - dot(n, r.o)) / denom;
+    if (t < near || t > far) return false;
+    const V3 p = r.o + r.d * t;
+    const V3 planar = p - pos_of(o);
+    const float alpha = dot(w, cross(planar, vvec(o)));
+    const float beta = dot(w, cross(uvec(o), planar));
+    if (tri) {
+        if (alpha < 0.0f || beta < 0.0f || alpha + beta > 1.0f) return false;
+    } else {
+        if (alpha < 0.0f || alpha > 1.0f || beta < 0.0f || beta > 1.0f) return false;
+    }
+    h.u = alpha;
+    h.v = beta;
+    h.t = t;
+    h.pos = p;
+    h.normal = unit;
+    h.face = dot(r.d, h.normal) < 0.0f;
+    if (!h.face) h.normal = -h.normal;
+    h.obj = id;
+    h.prim = ~0u;
+    return true;
+}
+
+static bool hit_box(const Object &o, uint32_t id, const Ray &r, float near, float far, Hit &h)
+{
+    const float ys = std::sin(o.rotation), yc = std::cos(o.rotation);
+    const V3 lo = rotate_y(r.o - pos_of(o), -ys, yc);
+    const V3 ld = rotate_y(r.d, -ys, yc);
+    const V3 half = v3(o.half_x, o.half_y, o.half_z);
+    const V3 inv = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+    const V3 t0 = (-half - lo) * inv;
+    const V3 t1 = (half - lo) * inv;
+    const V3 ts = vmin(t0, t1), tb = vmax(t0, t1);
+    const float tin = fmaxn(fmaxn(ts.x, ts.y), ts.z);
+    const float tout = fminn(fminn(tb.x, tb.y), tb.z);
+    V3 en = v3(0.0f, 0.0f, -sgn(ld.z));
+    if (tin == ts.x) en = v3(-sgn(ld.x), 0.0f, 0.0f);
+    else if (tin == ts.y) en = v3(0.0f, -sgn(ld.y), 0.0f);
+    V3 ex = v3(0.0f, 0.0f, sgn(ld.z));
+    if (tout == tb.x) ex = v3(sgn(ld.x), 0.0f, 0.0f);
+    else if (tout == tb.y) ex = v3(0.0f, sgn(ld.y), 0.0f);
+    if (!(tout > tin)) return false;
+    float t = tin;
+    V3 ln = en;
+    bool outside = true;
+    if (t <= near || far <= t) {
+        t = tout;
+        ln = ex;
+        outside = false;
+        if (t <= near || far <= t) return false;
+    }
+    h.t = t;
+    h.pos = r.o + r.d * t;
+    const V3 lp = lo + ld * t;
+    float uu, vv;
+    if (std::fabs(ln.x) > 0.5f) { uu = lp.z / half.z; vv = lp.y / half.y; }
+    else if (std::fabs(ln.y) > 0.5f) { uu = lp.x / half.x; vv = lp.z / half.z; }
+    else { uu = lp.x / half.x; vv = lp.y / half.y; }
+    h.u = saturate(uu * 0.5f + 0.5f);
+    h.v = saturate(vv * 0.5f + 0.5f);
+    h.normal = rotate_y(ln, ys, yc);
+    h.face = outside;
+    if (!h.face) h.normal = -h.normal;
+    h.obj = id;
+    h.prim = ~0u;
+    return true;
+}
+
+static inline V3 quat_rotate(float qx, float qy, float qz, float qw, V3 v)
+{
+    const V3 u = v3(qx, qy, qz);
+    const V3 t = 2.0f * cross(u, v);
+    return v + qw * t + cross(u, t);
+}
+
+static inline bool intersect_aabb(V3 o, V3 inv, const Node &n, float near, float far)
+{
+    const V3 t0 = (v3(n.min_x, n.min_y, n.min_z) - o) * inv;
+    const V3 t1 = (v3(n.max_x, n.max_y, n.max_z) - o) * inv;
+    const V3 ts = vmin(t0, t1), tb = vmax(t0, t1);
+    const float tmin = fmaxn(fmaxn(ts.x, ts.y), fmaxn(ts.z, near));
+    const float tmax = fminn(fminn(tb.x, tb.y), fminn(tb.z, far));
+    return tmin <= tmax;
+}
+
+static bool hit_mesh_tri(const Ray &r, float near, float far, const Vertex &a, const Vertex &b,
+                         const Vertex &c, float *t, float *alpha, float *beta, V3 *geo)
+{
+    const V3 p0 = v3(a.px, a.py, a.pz);
+    const V3 e1 = v3(b.px, b.py, b.pz) - p0;
+    const V3 e2 = v3(c.px, c.py, c.pz) - p0;
+    const V3 n = cross(e1, e2);
+    const float nl = dot(n, n);
+    if (nl < 1e-12f) return false;
+    const float denom = dot(n, r.d);
+    if (std::fabs(denom) < 1e-12f) return false;
+    *t = (dot(n, p0) - dot(n, r.o)) / denom;
+    if (*t < near || *t > far) return false;
+    const V3 w = n / nl;
+    const V3 planar = r.o + r.d * (*t) - p0;
+    *alpha = dot(w, cross(planar, e2));
+    *beta = dot(w, cross(e1, planar));
+    if (*alpha < 0.0f || *beta < 0.0f || *alpha + *beta > 1.0f) return false;
+    *geo = n / std::sqrt(nl);
+    return true;
+}
+
+static bool hit_mesh(const Scene &s, const Object &o, uint32_t id, const Ray &r, float near, float far,
+                     Hit &h, Counters *cnt)
+{
+    const Inst &in = s.inst[o.instanceIndex];
+    if (cnt) cnt->insts++;
+    const float scale = (in.scale == 0.0f) ? 1.0f : in.scale;
+    Ray lr;
+    lr.o = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.o - v3(in.px, in.py, in.pz));
+    lr.d = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.d);
+    if (scale != 1.0f) {
+        lr.o = lr.o / scale;
+        lr.d = lr.d / scale;
+    }
+    lr.time = r.time;
+    const V3 inv = v3(1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z);
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    bool found = false;
+    float closest = far, ba = 0.0f, bb = 0.0f;
+    uint32_t base = 0;
+    V3 geo = v3(0.0f, 0.0f, 1.0f);
+    while (sp > 0) {
+        const Node &n = s.blas[in.blasBase + stack[--sp]];
+        if (cnt) cnt->nodes++;
+        if (!intersect_aabb(lr.o, inv, n, near, closest)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; ++i) {
+                const uint32_t b3 = in.indexBase + static_cast<uint32_t>(n.left + i) * 3u;
+                const Vertex &va = s.verts[in.vertexBase + s.idx[b3 + 0]];
+                const Vertex &vb = s.verts[in.vertexBase + s.idx[b3 + 1]];
+                const Vertex &vc = s.verts[in.vertexBase + s.idx[b3 + 2]];
+                if (cnt) cnt->tris++;
+                float t, al, be;
+                V3 g;
+                if (hit_mesh_tri(lr, near, closest, va, vb, vc, &t, &al, &be, &g)) {
+                    closest = t;
+                    found = true;
+                    ba = al;
+                    bb = be;
+                    base = b3;
+                    geo = g;
+                }
+            }
+        } else {
+            stack[sp++] = n.left;
+            stack[sp++] = n.right;
+        }
+    }
+    if (!found) return false;
+    const Vertex &a = s.verts[in.vertexBase + s.idx[base + 0]];
+    const Vertex &b = s.verts[in.vertexBase + s.idx[base + 1]];
+    const Vertex &c = s.verts[in.vertexBase + s.idx[base + 2]];
+    const float w0 = 1.0f - ba - bb;
+    const V3 sl = normalize(w0 * v3(a.nx, a.ny, a.nz) + ba * v3(b.nx, b.ny, b.nz) + bb * v3(c.nx, c.ny, c.nz));
+    V3 gl = geo;
+    if (dot(gl, sl) < 0.0f) gl = -gl;
+    h.t = closest;
+    h.pos = r.o + r.d * closest;
+    h.normal = normalize(quat_rotate(in.qx, in.qy, in.qz, in.qw, sl));
+    h.face = dot(r.d, quat_rotate(in.qx, in.qy, in.qz, in.qw, gl)) < 0.0f;
+    if (!h.face) h.normal = -h.normal;
+    h.u = w0 * a.u + ba * b.u + bb * c.u;
+    h.v = w0 * a.v + ba * b.v + bb * c.v;
+    h.obj = id;
+    h.prim = (base - in.indexBase) / 3u;
+    return true;
+}
+
+static bool hit_object(const Scene &s, uint32_t id, const Ray &r, float near, float far, Hit &h, Counters *cnt)
+{
+    const Object &o = s.objects[id];
+    if (cnt) cnt->objs++;
+    switch (o.shapeType) {
+    case SPHERE: return hit_sphere(o, id, r, near, far, h);
+    case QUAD: return hit_planar(o, id, r, near, far, false, h);
+    case TRIANGLE: return hit_planar(o, id, r, near, far, true, h);
+    case MESH: return hit_mesh(s, o, id, r, near, far, h, cnt);
+    default: return hit_box(o, id, r, near, far, h);
+    }
+}
+
+// TraverseBVH: unordered DFS, push left then right (right popped first),
+// far starts at 1e6 and shrinks on every hit.
+static bool traverse(const Scene &s, const Ray &r, float near, Hit &hit, Counters *cnt)
+{
+    if (cnt) cnt->rays++;
+    if (s.nnode == 0) return false;
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    bool found = false;
+    float far = 1000000.0f;
+    while (sp > 0) {
+        const Node &n = s.bvh[stack[--sp]];
+        if (cnt) cnt->nodes++;
+        if (!intersect_aabb(r.o, inv, n, near, far)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; i++) {
+                Hit h;
+                if (hit_object(s, static_cast<uint32_t>(n.left + i), r, near, far, h, cnt)) {
+                    far = h.t;
+                    hit = h;
+                    found = true;
+                }
+            }
+        } else {
+            stack[sp++] = n.left;
+            stack[sp++] = n.right;
+        }
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// Texture sampler (bilinear, REPEAT, level 0, sRGB decode per texel).
+// ---------------------------------------------------------------------------
+static float g_srgb_lut[256];
+static bool g_lut_ready = false;
+static void init_lut()
+{
+    if (g_lut_ready) return;
+    for (int i = 0; i < 256; i++) {
+        const double c = i / 255.0;
+        const double l = c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4);
+        g_srgb_lut[i] = static_cast<float>(l);
+    }
+    g_lut_ready = true;
+}
+
+static inline int wrap(int i, int n)
+{
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+static V3 texture_color(const Scene &s, uint32_t layer, float u, float v)
+{
+    if (s.tex == nullptr || s.tl == 0) return v3(1.0f, 1.0f, 1.0f);
+    if (layer >= s.tl) layer = s.tl - 1;
+    if (!std::isfinite(u)) u = 0.0f;
+    if (!std::isfinite(v)) v = 0.0f;
+    const float x = u * static_cast<float>(s.tw) - 0.5f;
+    const float y = v * static_cast<float>(s.th) - 0.5f;
+    const float x0 = std::floor(x), y0 = std::floor(y);
+    const float fx = x - x0, fy = y - y0;
+    const int W = static_cast<int>(s.tw), H = static_cast<int>(s.th);
+    const int i0 = wrap(static_cast<int>(std::fmod(x0, static_cast<float>(W))), W);
+    const int j0 = wrap(static_cast<int>(std::fmod(y0, static_cast<float>(H))), H);
+    const int i1 = (i0 + 1) % W, j1 = (j0 + 1) % H;
+    const uint8_t *base = s.tex + static_cast<size_t>(layer) * W * H * 4;
+    auto tx = [&](int i, int j) {
+        const uint8_t *p = base + (static_cast<size_t>(j) * W + i) * 4;
+        return v3(g_srgb_lut[p[0]], g_srgb_lut[p[1]], g_srgb_lut[p[2]]);
+    };
+    const V3 t00 = tx(i0, j0), t10 = tx(i1, j0), t01 = tx(i0, j1), t11 = tx(i1, j1);
+    const V3 top = t00 + fx * (t10 - t00);
+    const V3 bot = t01 + fx * (t11 - t01);
+    return top + fy * (bot - top);
+}
+
+// ray_trace.comp.hybrid.hlsl:940-949
+static V3 surface_albedo_hybrid(const Scene &s, const Hit &h)
+{
+    const Object &o = s.objects[h.obj];
+    if (o.textureID == NO_TEXTURE) return albedo_of(o);
+    const V3 texel = texture_color(s, o.textureID, h.u, h.v);
+    return texel * lerp(v3(1.0f, 1.0f, 1.0f), albedo_of(o), o.textureTint);
+}
+
+// path_trace.comp.hlsl:675-683
+static V3 surface_albedo_pt(const Scene &s, const Hit &h)
+{
+    const Object &o = s.objects[h.obj];
+    V3 a = albedo_of(o);
+    if (o.textureID != NO_TEXTURE) a = a * texture_color(s, o.textureID, h.u, h.v);
+    return a;
+}
+
+// GetBounce: the direction part, shared by both shaders
+// (path_trace.comp.hlsl:699-763, ray_trace.comp.hybrid.hlsl:965-1033).
+static bool get_bounce(const Object &o, const Ray &r, const Hit &h, Rng &rng, V3 *bounce)
+{
+    *bounce = v3(0.0f, 0.0f, 0.0f);
+    switch (o.colorType) {
+    case LAMBERTIAN: {
+        const V3 rd = rng.direction();
+        *bounce = h.normal + rd;
+        if (dot(*bounce, *bounce) < 0.001f) *bounce = h.normal;
+        return true;
+    }
+    case METAL: {
+        *bounce = normalize(reflect(r.d, h.normal));
+        const V3 rd = rng.direction();
+        *bounce = *bounce + rd * o.fuzz;
+        return dot(*bounce, h.normal) > 0.0f;
+    }
+    case DIELECTRIC: {
+        const float ior = o.refraction;
+        const float ri = h.face ? (1.0f / ior) : ior;
+        const V3 dir = normalize(r.d);
+        const float cosT = fminn(dot(-dir, h.normal), 1.0f);
+        const float sinT = std::sqrt(fmaxn(0.0f, 1.0f - cosT * cosT));
+        const float q = (1.0f - ri) / (1.0f + ri);
+        const float r0 = q * q;
+        const float m = 1.0f - cosT;
+        const float m5 = ((m * m) * (m * m)) * m;
+        const float refl = r0 + (1.0f - r0) * m5;
+        if (ri * sinT > 1.0f || refl > rng.next()) *bounce = reflect(dir, h.normal);
+        else *bounce = refract(dir, h.normal, ri);
+        return true;
+    }
+    case ISOTROPIC:
+        *bounce = rng.direction();
+        return true;
+    default:
+        return false;
+    }
+}
+
+// ray_trace.comp.hybrid.hlsl:1057-1119
+static void sample_light(const Object &l, V3 from, Rng &rng, V3 *p, float *solid)
+{
+    *p = pos_of(l);
+    *solid = 0.0f;
+    if (l.shapeType == QUAD) {
+        const float r1 = rng.next();
+        const float r2 = rng.next();
+        *p = pos_of(l) + r1 * uvec(l) + r2 * vvec(l);
+        const V3 cuv = cross(uvec(l), vvec(l));
+        const float area = length(cuv);
+        const V3 ln = cuv / fmaxn(area, 1e-8f);
+        V3 ts = from - *p;
+        const float d2 = dot(ts, ts);
+        ts = normalize(ts);
+        const float cl = std::fabs(dot(ln, ts));
+        *solid = (cl * area) / fmaxn(d2, 1e-8f);
+        return;
+    }
+    if (l.shapeType == SPHERE) {
+        const V3 tl = pos_of(l) - from;
+        const float d2 = fmaxn(dot(tl, tl), 1e-8f);
+        const float r2 = l.radius * l.radius;
+        *p = pos_of(l);
+        if (d2 <= r2) {
+            *solid = 4.0f * 3.14159265f;
+            return;
+        }
+        const float ct = std::sqrt(fmaxn(0.0f, 1.0f - r2 / d2));
+        *solid = 2.0f * 3.14159265f * (1.0f - ct);
+        return;
+    }
+    const V3 tl = pos_of(l) - from;
+    const float d2 = fmaxn(dot(tl, tl), 1e-8f);
+    const float area = 4.0f * (l.half_x * l.half_y + l.half_y * l.half_z + l.half_z * l.half_x) / 6.0f;
+    *solid = area / d2;
+}
+
+static V3 direct_light(const Scene &s, const Hit &hit, const Ray &ray, Rng &rng, Counters *cnt)
+{
+    // cnt points at {extension counters, shadow counters}; shadow rays count
+    // in the second set.
+    Counters *shadow_cnt = cnt ? cnt + 1 : nullptr;
+    V3 sum = v3(0.0f, 0.0f, 0.0f);
+    if (s.nlights == 0) return sum;
+    for (uint32_t i = 0; i < s.nlights; i++) {
+        const Object &l = s.objects[s.lights[i]];
+        V3 lp;
+        float solid;
+        sample_light(l, hit.pos, rng, &lp, &solid);
+        if (solid <= 0.0f) continue;
+        V3 dir = lp - hit.pos;
+        const float dist = length(dir);
+        if (dist <= 1e-6f) continue;
+        dir = dir / dist;
+        const float ct = dot(hit.normal, dir);
+        if (ct <= 0.0f) continue;
+        Ray sr;
+        sr.o = hit.pos + (hit.normal * 0.001f);
+        sr.d = dir;
+        sr.time = ray.time;
+        bool occluded = false;
+        for (int step = 0; step < 4; step++) {
+            Hit sh;
+            if (!traverse(s, sr, 0.001f, sh, shadow_cnt)) break;
+            if (sh.t >= length(lp - sr.o) - 0.001f) break;
+            const uint32_t ct2 = s.objects[sh.obj].colorType;
+            if (ct2 == DIFFUSE_LIGHT) break;
+            if (ct2 != DIELECTRIC) {
+                occluded = true;
+                break;
+            }
+            sr.o = sh.pos + (sr.d * 0.001f);
+        }
+        if (!occluded) {
+            const V3 radiance = l.emission * albedo_of(l);
+            const V3 brdf = surface_albedo_hybrid(s, hit);
+            sum = sum + brdf * radiance * ct * solid;
+        }
+    }
+    return sum;
+}
+
+// ---------------------------------------------------------------------------
+// Camera
+// ---------------------------------------------------------------------------
+struct Camera {
+    V3 source, pixel0, pixelU, pixelV, du, dv;
+    float defocus;
+};
+
+static Camera make_camera(const Config &c)
+{
+    Camera cam;
+    const V3 src = v3(c.source[0], c.source[1], c.source[2]);
+    const V3 tgt = v3(c.target[0], c.target[1], c.target[2]);
+    const V3 up = v3(c.up[0], c.up[1], c.up[2]);
+    const V3 w = normalize(src - tgt);
+    const V3 u = normalize(cross(up, w));
+    const V3 v = cross(w, u);
+    const float rad = 0.0174532925f;
+    const float vh = 2.0f * std::tan((c.fov / 2.0f) * rad) * c.focus;
+    const float vw = vh * static_cast<float>(c.width) / static_cast<float>(c.height);
+    const V3 vu = vw * u;
+    const V3 vv = vh * -v;
+    cam.pixelU = vu / static_cast<float>(c.width);
+    cam.pixelV = vv / static_cast<float>(c.height);
+    cam.pixel0 = src - c.focus * w - vu / 2.0f - vv / 2.0f + cam.pixelU / 2.0f + cam.pixelV / 2.0f;
+    const float dr = c.focus * std::tan((c.defocus / 2.0f) * rad);
+    cam.du = u * dr;
+    cam.dv = v * dr;
+    cam.source = src;
+    cam.defocus = c.defocus;
+    return cam;
+}
+
+static Ray camera_ray(const Camera &cam, uint32_t x, uint32_t y, Rng &rng)
+{
+    const float jx = rng.next() - 0.5f;
+    const float jy = rng.next() - 0.5f;
+    const V3 target = cam.pixel0 + (static_cast<float>(x) + jx) * cam.pixelU + (static_cast<float>(y) + jy) * cam.pixelV;
+    Ray r;
+    if (cam.defocus <= 0.0f) {
+        r.o = cam.source;
+    } else {
+        float px, py;
+        while (true) {
+            px = rng.next() * 2.0f - 1.0f;
+            py = rng.next() * 2.0f - 1.0f;
+            if (std::sqrt(px * px + py * py + 0.0f * 0.0f) < 1.0f) break;
+        }
+        r.o = cam.source + (px * cam.du) + (py * cam.dv);
+    }
+    r.d = normalize(target - r.o);
+    r.time = rng.next();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Integrators
+// ---------------------------------------------------------------------------
+static V3 color_ray_hybrid(const Scene &s, Ray ray, uint32_t depth, const V3 &sky, Rng &rng, Counters *cnt)
+{
+    V3 tp = v3(1.0f, 1.0f, 1.0f);
+    V3 acc = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t b = 0; b < depth; b++) {
+        Hit hit;
+        if (!traverse(s, ray, 0.001f, hit, cnt)) {
+            acc = acc + tp * sky;
+            break;
+        }
+        const Object &o = s.objects[hit.obj];
+        if (o.colorType == DIFFUSE_LIGHT) {
+            acc = acc + tp * (o.emission * albedo_of(o));
+            break;
+        }
+        if (o.colorType == LAMBERTIAN) {
+            V3 dl = direct_light(s, hit, ray, rng, cnt);
+            const float luma = fmaxn(dl.x, fmaxn(dl.y, dl.z));
+            if (luma > 8.0f) dl = dl * (8.0f / luma);
+            acc = acc + tp * dl;
+        }
+        V3 sc;
+        if (!get_bounce(o, ray, hit, rng, &sc)) break;
+        ray.d = normalize(sc);
+        const float sign = (dot(ray.d, hit.normal) < 0.0f) ? -1.0f : 1.0f;
+        ray.o = hit.pos + hit.normal * (0.001f * sign);
+        if (o.colorType != DIELECTRIC) tp = tp * surface_albedo_hybrid(s, hit);
+        if (b > 0) {
+            float p = fmaxn(tp.x, fmaxn(tp.y, tp.z));
+            p = fminn(fmaxn(p, 0.1f), 1.0f);
+            if (rng.next() > p) break;
+            tp = tp / p;
+        }
+    }
+    return acc;
+}
+
+static V3 color_ray_pt(const Scene &s, Ray ray, uint32_t depth, const V3 &sky, const V3 &horizon, Rng &rng,
+                       Counters *cnt)
+{
+    V3 tp = v3(1.0f, 1.0f, 1.0f);
+    V3 color = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t b = 0; b < depth; b++) {
+        Hit hit;
+        if (!traverse(s, ray, 0.001f, hit, cnt)) {
+            const float t = 0.5f * (normalize(ray.d).y + 1.0f);
+            color = color + tp * lerp(horizon, sky, t);
+            break;
+        }
+        const Object &o = s.objects[hit.obj];
+        const V3 att = surface_albedo_pt(s, hit);
+        if (o.colorType == DIFFUSE_LIGHT) color = color + tp * (att * o.emission);
+        V3 bounce;
+        if (!get_bounce(o, ray, hit, rng, &bounce)) break;
+        ray.o = hit.pos;
+        ray.d = bounce;
+        tp = tp * att;
+    }
+    return color;
+}
+
+static inline float aces(float x)
+{
+    return saturate((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f));
+}
+static inline float to_srgb(float c)
+{
+    c = saturate(c);
+    if (c <= 0.0031308f) return c * 12.92f;
+    return 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f;
+}
+static inline uint8_t unorm8(float x)
+{
+    return static_cast<uint8_t>(std::floor(saturate(x) * 255.0f + 0.5f));
+}
+
+static inline float round_half(float x)
+{
+    // Round to the nearest IEEE half (round-half-even), back to float.
+    if (!std::isfinite(x)) return x;
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u;
+    const float ax = std::fabs(x);
+    if (ax >= 65520.0f) return sign ? -INFINITY : INFINITY;
+    if (ax < 6.103515625e-05f) {
+        // subnormal half: quantum 2^-24
+        const float q = 5.9604644775390625e-08f;
+        const float r = std::nearbyint(ax / q) * q;
+        return sign ? -r : r;
+    }
+    uint32_t a;
+    std::memcpy(&a, &ax, 4);
+    const uint32_t lsb = (a >> 13) & 1u;
+    a = a + 0x0FFFu + lsb;
+    a &= ~0x1FFFu;
+    float r;
+    std::memcpy(&r, &a, 4);
+    return sign ? -r : r;
+}
+
+} // namespace orc
+
+using namespace orc;
+
+static Scene make_scene(const Object *objects, uint32_t nobj, const Node *bvh, uint32_t nnode,
+                        const uint32_t *lights, uint32_t nlights, const Vertex *verts, const uint32_t *idx,
+                        const Node *blas, const Inst *inst, const uint8_t *tex, uint32_t tw, uint32_t th,
+                        uint32_t tl)
+{
+    init_lut();
+    Scene s;
+    s.objects = objects;
+    s.nobj = nobj;
+    s.bvh = bvh;
+    s.nnode = nnode;
+    s.lights = lights;
+    s.nlights = nlights;
+    s.verts = verts;
+    s.idx = idx;
+    s.blas = blas;
+    s.inst = inst;
+    s.tex = tex;
+    s.tw = tw;
+    s.th = th;
+    s.tl = tl;
+    return s;
+}
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// BVH builder (scene/bvh.h:58-295), recursion and std::partition as written.
+// ---------------------------------------------------------------------------
+static inline AABB surround(const AABB &a, const AABB &b)
+{
+    return AABB{std::min(a.min_x, b.min_x), std::min(a.min_y, b.min_y), std::min(a.min_z, b.min_z),
+                std::max(a.max_x, b.max_x), std::max(a.max_y, b.max_y), std::max(a.max_z, b.max_z)};
+}
+static inline AABB empty_box() { return AABB{FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX}; }
+static inline float centroid(const AABB &b, int axis)
+{
+    return axis == 0 ? (b.min_x + b.max_x) * 0.5f : axis == 1 ? (b.min_y + b.max_y) * 0.5f : (b.min_z + b.max_z) * 0.5f;
+}
+static inline float half_area(const AABB &b)
+{
+    const float dx = b.max_x - b.min_x, dy = b.max_y - b.min_y, dz = b.max_z - b.min_z;
+    return dx * dy + dy * dz + dz * dx;
+}
+static inline int sah_bin(float c, float cmin, float scale)
+{
+    return std::min(11, static_cast<int>((c - cmin) * scale));
+}
+static Node make_node(const AABB &b, int left, int right, int count)
+{
+    const float bm = 0.0001f;
+    Node n{};
+    n.min_x = b.min_x; n.min_y = b.min_y; n.min_z = b.min_z;
+    n.max_x = b.max_x; n.max_y = b.max_y; n.max_z = b.max_z;
+    if (n.max_x - n.min_x < bm) { n.min_x -= bm * 0.5f; n.max_x += bm * 0.5f; }
+    if (n.max_y - n.min_y < bm) { n.min_y -= bm * 0.5f; n.max_y += bm * 0.5f; }
+    if (n.max_z - n.min_z < bm) { n.min_z -= bm * 0.5f; n.max_z += bm * 0.5f; }
+    n.left = left;
+    n.right = right;
+    n.count = count;
+    return n;
+}
+
+static int build_rec(std::vector<uint32_t> &idx, int start, int end, const AABB *bounds, std::vector<Node> &nodes,
+                     std::vector<uint32_t> &order, int depth)
+{
+    AABB box = empty_box();
+    float cmin[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmax[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = start; i < end; i++) {
+        const AABB &b = bounds[idx[i]];
+        box = surround(box, b);
+        for (int a = 0; a < 3; a++) {
+            const float c = centroid(b, a);
+            cmin[a] = std::min(cmin[a], c);
+            cmax[a] = std::max(cmax[a], c);
+        }
+    }
+    const int me = static_cast<int>(nodes.size());
+    nodes.push_back(Node{});
+    const int count = end - start;
+    if (count <= 4 || depth >= 20) {
+        const int first = static_cast<int>(order.size());
+        for (int i = start; i < end; i++) order.push_back(idx[i]);
+        nodes[me] = make_node(box, first, -1, count);
+        return me;
+    }
+    int bestAxis = -1, bestSplit = 0;
+    float bestCost = FLT_MAX;
+    for (int axis = 0; axis < 3; axis++) {
+        const float extent = cmax[axis] - cmin[axis];
+        if (extent <= 0.0f) continue;
+        const float scale = 12.0f / extent;
+        AABB bb[12];
+        int bc[12] = {};
+        for (int b = 0; b < 12; b++) bb[b] = empty_box();
+        for (int i = start; i < end; i++) {
+            const AABB &p = bounds[idx[i]];
+            const int b = sah_bin(centroid(p, axis), cmin[axis], scale);
+            bb[b] = surround(bb[b], p);
+            bc[b]++;
+        }
+        float lc[11];
+        int ln[11];
+        AABB acc = empty_box();
+        int an = 0;
+        for (int s = 0; s < 11; s++) {
+            acc = surround(acc, bb[s]);
+            an += bc[s];
+            ln[s] = an;
+            lc[s] = an > 0 ? half_area(acc) * an : 0.0f;
+        }
+        acc = empty_box();
+        an = 0;
+        for (int s = 11; s > 0; s--) {
+            acc = surround(acc, bb[s]);
+            an += bc[s];
+            if (ln[s - 1] == 0 || an == 0) continue;
+            const float cost = lc[s - 1] + half_area(acc) * an;
+            if (cost < bestCost) {
+                bestCost = cost;
+                bestAxis = axis;
+                bestSplit = s;
+            }
+        }
+    }
+    int mid = start + count / 2;
+    if (bestAxis >= 0) {
+        const float scale = 12.0f / (cmax[bestAxis] - cmin[bestAxis]);
+        auto m = std::partition(idx.begin() + start, idx.begin() + end, [&](uint32_t i) {
+            return sah_bin(centroid(bounds[i], bestAxis), cmin[bestAxis], scale) < bestSplit;
+        });
+        mid = static_cast<int>(m - idx.begin());
+    }
+    if (mid <= start || mid >= end) mid = start + count / 2;
+    const int l = build_rec(idx, start, mid, bounds, nodes, order, depth + 1);
+    const int r = build_rec(idx, mid, end, bounds, nodes, order, depth + 1);
+    nodes[me] = make_node(box, l, r, 0);
+    return me;
+}
+
+// Returns the node count; nodes must hold 2n entries, order n entries.
+int orc_build_bvh(const AABB *bounds, uint32_t n, Node *out_nodes, uint32_t *out_order)
+{
+    if (n == 0) return 0;
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::vector<Node> nodes;
+    std::vector<uint32_t> order;
+    nodes.reserve(2 * n);
+    order.reserve(n);
+    build_rec(idx, 0, static_cast<int>(n), bounds, nodes, order, 0);
+    std::memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(Node));
+    std::memcpy(out_order, order.data(), order.size() * sizeof(uint32_t));
+    return static_cast<int>(nodes.size());
+}
+
+// MeshLibrary::Add's per-mesh part (scene/mesh_library.h:104-155): vertex
+// bounds, triangle AABBs (Surround2Points then SurroundPoint), BuildBVH and
+// the index buffer in BLAS slot order. Returns the node count.
+int orc_build_blas(const float *pos, uint32_t nv, const uint32_t *tri, uint32_t ntri, Node *out_nodes,
+                   uint32_t *out_idx, AABB *out_local)
+{
+    AABB local = empty_box();
+    for (uint32_t i = 0; i < nv; i++) {
+        const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        local = AABB{std::min(local.min_x, x), std::min(local.min_y, y), std::min(local.min_z, z),
+                     std::max(local.max_x, x), std::max(local.max_y, y), std::max(local.max_z, z)};
+    }
+    *out_local = local;
+    std::vector<AABB> tb(ntri);
+    for (uint32_t t = 0; t < ntri; t++) {
+        const float *a = pos + 3 * tri[3 * t], *b = pos + 3 * tri[3 * t + 1], *c = pos + 3 * tri[3 * t + 2];
+        AABB e{std::min(a[0], b[0]), std::min(a[1], b[1]), std::min(a[2], b[2]),
+               std::max(a[0], b[0]), std::max(a[1], b[1]), std::max(a[2], b[2])};
+        tb[t] = AABB{std::min(e.min_x, c[0]), std::min(e.min_y, c[1]), std::min(e.min_z, c[2]),
+                     std::max(e.max_x, c[0]), std::max(e.max_y, c[1]), std::max(e.max_z, c[2])};
+    }
+    std::vector<uint32_t> order(ntri);
+    const int nn = ntri ? orc_build_bvh(tb.data(), ntri, out_nodes, order.data()) : 0;
+    for (uint32_t s = 0; s < ntri; s++) {
+        out_idx[3 * s + 0] = tri[3 * order[s] + 0];
+        out_idx[3 * s + 1] = tri[3 * order[s] + 1];
+        out_idx[3 * s + 2] = tri[3 * order[s] + 2];
+    }
+    return nn;
+}
+
+// MeshBounds (scene/shapes.h:124-145) with the uniform-scale extension:
+// corner * scale, rotated by q, plus position.
+void orc_instance_bounds(const Inst *inst, uint32_t n, const AABB *local, AABB *out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const AABB &b = local[i];
+        if (b.min_x > b.max_x) {
+            out[i] = empty_box();
+            continue;
+        }
+        const float s = inst[i].scale == 0.0f ? 1.0f : inst[i].scale;
+        AABB w = empty_box();
+        for (int c = 0; c < 8; c++) {
+            V3 p = v3((c & 1) ? b.max_x : b.min_x, (c & 2) ? b.max_y : b.min_y, (c & 4) ? b.max_z : b.min_z);
+            if (s != 1.0f) p = p * s;
+            const V3 r = quat_rotate(inst[i].qx, inst[i].qy, inst[i].qz, inst[i].qw, p);
+            const V3 q = v3(inst[i].px, inst[i].py, inst[i].pz) + r;
+            w = AABB{std::min(w.min_x, q.x), std::min(w.min_y, q.y), std::min(w.min_z, q.z),
+                     std::max(w.max_x, q.x), std::max(w.max_y, q.y), std::max(w.max_z, q.z)};
+        }
+        out[i] = w;
+    }
+}
+
+void orc_sincos(float a, float *s, float *c) { rt_sincos(a, s, c); }
+
+float orc_random(uint32_t *seed)
+{
+    Rng r{*seed};
+    const float v = r.next();
+    *seed = r.seed;
+    return v;
+}
+
+float orc_round_half(float x) { return round_half(x); }
+
+void orc_srgb_lut(float *out)
+{
+    init_lut();
+    std::memcpy(out, g_srgb_lut, sizeof(g_srgb_lut));
+}
+
+// Camera basis as the host computes it: pixel0, pixelU, pixelV, du, dv (15 floats).
+void orc_camera(const Config *cfg, float *out15)
+{
+    const Camera c = make_camera(*cfg);
+    const V3 v[5] = {c.pixel0, c.pixelU, c.pixelV, c.du, c.dv};
+    for (int i = 0; i < 5; i++) {
+        out15[3 * i] = v[i].x;
+        out15[3 * i + 1] = v[i].y;
+        out15[3 * i + 2] = v[i].z;
+    }
+}
+
+// Primary visibility for frame `cfg->batch` (first sample of each pixel):
+// hit object slot, mesh triangle slot (BLAS slot) and t, plus the ray itself
+// (origin xyz, dir xyz) when rays_out is not null. Pixels [p0, p1).
+void orc_trace_primary(const Config *cfg, const Object *objects, uint32_t nobj, const Node *bvh, uint32_t nnode,
+                       const Vertex *verts, const uint32_t *idx, const Node *blas, const Inst *inst,
+                       uint32_t p0, uint32_t p1, uint32_t *hit_obj, uint32_t *hit_prim, float *hit_t,
+                       float *rays_out)
+{
+    Scene s = make_scene(objects, nobj, bvh, nnode, nullptr, 0, verts, idx, blas, inst, nullptr, 0, 0, 0);
+    const Camera cam = make_camera(*cfg);
+    const uint32_t W = cfg->width, H = cfg->height;
+    for (uint32_t p = p0; p < p1; p++) {
+        const uint32_t x = p % W, y = p / W;
+        Rng rng{x + y * W + cfg->batch * W * H + 1u};
+        const Ray r = camera_ray(cam, x, y, rng);
+        Hit h;
+        const uint32_t k = p - p0;
+        if (traverse(s, r, 0.001f, h, nullptr)) {
+            hit_obj[k] = h.obj;
+            hit_prim[k] = h.prim;
+            hit_t[k] = h.t;
+        } else {
+            hit_obj[k] = ~0u;
+            hit_prim[k] = ~0u;
+            hit_t[k] = 0.0f;
+        }
+        if (rays_out) {
+            float *o = rays_out + 6 * static_cast<size_t>(k);
+            o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z;
+            o[3] = r.d.x; o[4] = r.d.y; o[5] = r.d.z;
+        }
+    }
+}
+
+// Full render of `samples` reference frames (Batch = batch .. batch+samples-1,
+// Samples = 1 each) for the rows [row0, row1), accumulated in frame order into
+// `image` (RGBA32F, full-size, previous contents used when batch > 0) and, in
+// ray-trace mode, `display` (RGBA8). mode 0 = path_trace, 1 = ray_trace.
+// `threads` workers pull rows from a shared counter. fp16 != 0 rounds the
+// accumulation through half after every frame (ray_trace mode's RGBA16F
+// target). Counters are written to stats[10] = {rays, nodes, objects,
+// instances, triangles} for extension (camera and bounce) rays, then the same
+// five for shadow rays, when not null. pixel_stride > 1 renders (and counts)
+// only pixels whose index is a multiple of it. Returns elapsed seconds.
+double orc_render(const Config *cfg, int mode, const Object *objects, uint32_t nobj, const Node *bvh,
+                  uint32_t nnode, const uint32_t *lights, uint32_t nlights, const Vertex *verts,
+                  const uint32_t *idx, const Node *blas, const Inst *inst, const uint8_t *tex, uint32_t tw,
+                  uint32_t th, uint32_t tl, uint32_t row0, uint32_t row1, int threads, int fp16, float *image,
+                  uint8_t *display, uint64_t *stats, uint32_t pixel_stride)
+{
+    Scene s = make_scene(objects, nobj, bvh, nnode, lights, nlights, verts, idx, blas, inst, tex, tw, th, tl);
+    const Camera cam = make_camera(*cfg);
+    const uint32_t W = cfg->width, H = cfg->height;
+    const uint32_t S = cfg->samples ? cfg->samples : 1u;
+    const V3 sky = v3(cfg->sky[0], cfg->sky[1], cfg->sky[2]);
+    const V3 hor = v3(cfg->horizon[0], cfg->horizon[1], cfg->horizon[2]);
+    std::atomic<uint32_t> next_row{row0};
+    std::vector<Counters> per(2 * (threads > 0 ? threads : 1));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&](int tid) {
+        Counters local[2];
+        while (true) {
+            const uint32_t y = next_row.fetch_add(1);
+            if (y >= row1) break;
+            for (uint32_t x = 0; x < W; x++) {
+                const uint32_t p = y * W + x;
+                if (pixel_stride > 1 && p % pixel_stride != 0) continue;
+                Counters *cnt = stats ? local : nullptr;
+                float acc[3] = {image[4 * p], image[4 * p + 1], image[4 * p + 2]};
+                for (uint32_t sidx = 0; sidx < S; sidx++) {
+                    const uint32_t batch = cfg->batch + sidx;
+                    Rng rng{x + y * W + batch * W * H + 1u};
+                    const Ray r = camera_ray(cam, x, y, rng);
+                    V3 c;
+                    if (mode == 0) c = color_ray_pt(s, r, cfg->depth, sky, hor, rng, cnt);
+                    else c = color_ray_hybrid(s, r, cfg->depth, sky, rng, cnt);
+                    // Samples / Batches == 1 per frame: color /= 1 is exact.
+                    if (mode == 0) {
+                        const V3 prev = batch > 0 ? v3(acc[0], acc[1], acc[2]) : v3(0.0f, 0.0f, 0.0f);
+                        const V3 a = (prev * static_cast<float>(batch) + c) / static_cast<float>(batch + 1u);
+                        acc[0] = a.x; acc[1] = a.y; acc[2] = a.z;
+                    } else {
+                        V3 a;
+                        if (batch == 0) {
+                            a = c;
+                        } else {
+                            const V3 prev = v3(acc[0], acc[1], acc[2]);
+                            const float w = 1.0f / static_cast<float>(batch + 1u);
+                            a = lerp(prev, c, w);
+                        }
+                        if (fp16) a = v3(round_half(a.x), round_half(a.y), round_half(a.z));
+                        acc[0] = a.x; acc[1] = a.y; acc[2] = a.z;
+                    }
+                }
+                image[4 * p] = acc[0];
+                image[4 * p + 1] = acc[1];
+                image[4 * p + 2] = acc[2];
+                image[4 * p + 3] = 1.0f;
+                if (display && mode == 1) {
+                    const float e = cfg->exposure;
+                    display[4 * p + 0] = unorm8(to_srgb(aces(acc[0] * e)));
+                    display[4 * p + 1] = unorm8(to_srgb(aces(acc[1] * e)));
+                    display[4 * p + 2] = unorm8(to_srgb(aces(acc[2] * e)));
+                    display[4 * p + 3] = 255;
+                }
+            }
+        }
+        per[2 * tid] = local[0];
+        per[2 * tid + 1] = local[1];
+    };
+    const int nt = threads > 0 ? threads : 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &t : pool) t.join();
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+        for (int k = 0; k < 2; k++) {
+            Counters tot;
+            for (size_t i = k; i < per.size(); i += 2) {
+                const Counters &c = per[i];
+                tot.rays += c.rays; tot.nodes += c.nodes; tot.objs += c.objs; tot.insts += c.insts; tot.tris += c.tris;
+            }
+            stats[5 * k + 0] = tot.rays; stats[5 * k + 1] = tot.nodes; stats[5 * k + 2] = tot.objs;
+            stats[5 * k + 3] = tot.insts; stats[5 * k + 4] = tot.tris;
+        }
+    }
+    (void)H;
+    return secs;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 intersection oracle: closest hit over every primitive in double
+// precision (brute force, no BVH), for rays given in FP32. Spheres use the
+// quadratic with the same near/far rules; triangles (objects and mesh
+// triangles, instance transform applied to the vertices in double) use the
+// plane/barycentric test. Writes object slot, mesh triangle slot and t.
+// ---------------------------------------------------------------------------
+struct D3 {
+    double x, y, z;
+};
+static inline D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+static inline D3 dsub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline D3 dadd(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline D3 dmul(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+static inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline D3 dcross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline D3 drot(const Inst &q, D3 v)
+{
+    const D3 u = d3(q.qx, q.qy, q.qz);
+    const D3 t = dmul(dcross(u, v), 2.0);
+    return dadd(dadd(v, dmul(t, q.qw)), dcross(u, t));
+}
+
+static bool dtri(D3 o, D3 d, D3 p0, D3 p1, D3 p2, double near, double far, double *t)
+{
+    const D3 e1 = dsub(p1, p0), e2 = dsub(p2, p0);
+    const D3 n = dcross(e1, e2);
+    const double nl = ddot(n, n);
+    if (nl < 1e-24) return false;
+    const double den = ddot(n, d);
+    if (std::fabs(den) < 1e-24) return false;
+    const double tt = (ddot(n, p0) - ddot(n, o)) / den;
+    if (tt < near || tt > far) return false;
+    const D3 pl = dsub(dadd(o, dmul(d, tt)), p0);
+    const D3 w = dmul(n, 1.0 / nl);
+    const double a = ddot(w, dcross(pl, e2)), b = ddot(w, dcross(e1, pl));
+    if (a < 0.0 || b < 0.0 || a + b > 1.0) return false;
+    *t = tt;
+    return true;
+}
+
+void orc_closest_fp64(const float *rays, uint32_t nrays, const Object *objects, uint32_t nobj,
+                      const Vertex *verts, const uint32_t *idx, const Inst *inst, uint32_t *hit_obj,
+                      uint32_t *hit_prim, double *hit_t)
+{
+    for (uint32_t k = 0; k < nrays; k++) {
+        const D3 o = d3(rays[6 * k], rays[6 * k + 1], rays[6 * k + 2]);
+        const D3 d = d3(rays[6 * k + 3], rays[6 * k + 4], rays[6 * k + 5]);
+        double best = 1e6;
+        uint32_t bo = ~0u, bp = ~0u;
+        for (uint32_t i = 0; i < nobj; i++) {
+            const Object &ob = objects[i];
+            if (ob.shapeType == SPHERE) {
+                const D3 c = d3(ob.x, ob.y, ob.z);
+                const D3 off = dsub(c, o);
+                const double a = ddot(d, d), b = ddot(d, off), cc = ddot(off, off) - (double)ob.radius * ob.radius;
+                const double disc = b * b - a * cc;
+                if (disc < 0.0) continue;
+                const double sq = std::sqrt(disc);
+                double root = (b - sq) / a;
+                if (root <= 0.001 || best <= root) {
+                    root = (b + sq) / a;
+                    if (root <= 0.001 || best <= root) continue;
+                }
+                best = root; bo = i; bp = ~0u;
+            } else if (ob.shapeType == TRIANGLE || ob.shapeType == QUAD) {
+                const D3 p0 = d3(ob.x, ob.y, ob.z);
+                const D3 u = d3(ob.u_x, ob.u_y, ob.u_z), v = d3(ob.v_x, ob.v_y, ob.v_z);
+                double t;
+                if (ob.shapeType == TRIANGLE) {
+                    if (dtri(o, d, p0, dadd(p0, u), dadd(p0, v), 0.001, best, &t)) { best = t; bo = i; bp = ~0u; }
+                } else {
+                    if (dtri(o, d, p0, dadd(p0, u), dadd(p0, v), 0.001, best, &t) ||
+                        dtri(o, d, dadd(dadd(p0, u), v), dadd(p0, v), dadd(p0, u), 0.001, best, &t)) {
+                        best = t; bo = i; bp = ~0u;
+                    }
+                }
+            } else if (ob.shapeType == MESH) {
+                const Inst &q = inst[ob.instanceIndex];
+                const double s = q.scale == 0.0f ? 1.0 : q.scale;
+                for (uint32_t tri = 0; tri < q.triangleCount; tri++) {
+                    D3 p[3];
+                    for (int c = 0; c < 3; c++) {
+                        const Vertex &vv = verts[q.vertexBase + idx[q.indexBase + 3 * tri + c]];
+                        p[c] = dadd(drot(q, dmul(d3(vv.px, vv.py, vv.pz), s)), d3(q.px, q.py, q.pz));
+                    }
+                    double t;
+                    if (dtri(o, d, p[0], p[1], p[2], 0.001, best, &t)) { best = t; bo = i; bp = tri; }
+                }
+            }
+        }
+        hit_obj[k] = bo;
+        hit_prim[k] = bp;
+        hit_t[k] = bo == ~0u ? 0.0 : best;
+    }
+}
+
+int orc_sizeof_config(void) { return static_cast<int>(sizeof(Config)); }
+
+} // extern "C"
