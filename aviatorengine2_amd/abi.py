@@ -98,7 +98,7 @@ class AvrtStats(ctypes.Structure):
         ("rays_shadow", ctypes.c_uint64), ("paths", ctypes.c_uint64),
         ("ms_total", ctypes.c_float), ("ms_extend", ctypes.c_float), ("ms_shade", ctypes.c_float),
         ("ms_shadow", ctypes.c_float), ("ms_other", ctypes.c_float),
-        ("launches_extend", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3),
+        ("launches_extend", ctypes.c_uint32), ("ms_scene_step", ctypes.c_float), ("reserved", ctypes.c_uint32 * 2),
     ]
 
     def as_dict(self):

@@ -1,0 +1,108 @@
+// avrt_ctx.h - the context behind the C ABI, shared by the render and scene
+// translation units of libavrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/avrt.h"
+#include "avrt_internal.h"
+
+namespace avrt {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    template <typename T>
+    T *as() const
+    {
+        return static_cast<T *>(p);
+    }
+};
+
+// Device state of the on-device scene rebuild (avrt_scene_*).
+struct SceneRebuild {
+    bool ready = false;
+    uint32_t n = 0;          // authoring objects
+    uint32_t m = 0;          // mesh instances
+    uint32_t nlights = 0, nquad_lights = 0;
+    DevBuf auth_objects;     // AvrtObject[n], authoring order
+    DevBuf auth_inst;        // AvrtMeshInstance[m]
+    DevBuf local;            // AvrtAABB[m]
+    DevBuf inst_geomc;       // uint4[m]: blas super-root, tri base, vertex base, index base
+    DevBuf mesh_rank;        // int32[n]: rank among meshes or -1
+    DevBuf vel;              // float[6m]
+    DevBuf bounds;           // AvrtAABB[n]
+    DevBuf nodes;            // AvrtBVHNode[2n]
+    DevBuf order;            // uint32[n]
+    DevBuf scratch;          // builder work arrays
+    DevBuf info;             // uint32[8]: node count, error flags
+    size_t scratch_bytes = 0;
+    float last_ms = 0.0f;
+};
+
+} // namespace avrt
+
+struct avrt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    AvrtCreateOptions opts{};
+    bool ref_order = false;
+
+    // geometry (host copies drive the per-mesh conversion)
+    std::vector<AvrtVertex> h_verts;
+    std::vector<uint32_t> h_indices;
+    std::vector<AvrtBVHNode> h_blas;
+    std::vector<avrt::Node2> h_blas2;
+    std::vector<avrt::TriPre> h_tris;
+    std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
+    std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase
+    bool geometry_loaded = false;
+    avrt::DevBuf d_verts, d_indices, d_blas2, d_tris;
+
+    // scene as the renderer reads it (slot order)
+    uint32_t nobj = 0, nlights = 0, nquad_lights = 0, ntlas2 = 0;
+    bool has_volume = false, has_tri_or_mesh = false;
+    bool scene_loaded = false;
+    avrt::DevBuf d_objects, d_geom, d_tlas2, d_lights, d_instances;
+
+    // textures
+    avrt::DevBuf d_tex;
+    uint32_t tw = 0, th = 0, tl = 0;
+
+    // frame targets
+    avrt::DevBuf d_image, d_display;
+    uint32_t img_w = 0, img_h = 0;
+
+    // path buffers
+    avrt::DevBuf so[2], sd[2], st[2], hitA, hitB, Lpath, shp, shn, sha, shw, counters;
+    size_t cap_paths = 0;
+
+    avrt::SceneRebuild rb;
+
+    AvrtStats stats{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_a = nullptr, ev_b = nullptr;
+};
+
+namespace avrt {
+
+int set_err(avrt_ctx *c, int code, const std::string &msg);
+int ensure(avrt_ctx *c, DevBuf &b, size_t bytes);
+void release(DevBuf &b);
+int upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes);
+// BLAS trees and triangle records for every mesh the instances use.
+int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ninst, bool *changed);
+int upload_mesh_tables(avrt_ctx *c);
+
+} // namespace avrt
+
+#define HIPX(ctx, call)                                                                                          \
+    do {                                                                                                         \
+        hipError_t e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                    \
+            return ::avrt::set_err((ctx), AVRT_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_));       \
+    } while (0)

@@ -21,6 +21,7 @@
 
 #include <cstdint>
 
+#include "../../include/avrt.h"
 #include "avrt_internal.h"
 
 namespace avrt {
@@ -189,6 +190,8 @@ __device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy,
     return tmin <= tmax;
 }
 
+constexpr uint32_t kMaxTraversalSteps = 1u << 24;
+
 struct HitRec {
     float t;
     uint32_t obj;   // TLAS slot, ~0u on a miss
@@ -316,7 +319,10 @@ __device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, fl
     bool found = false;
     uint32_t ref = 0; // TLAS super-root
     uint32_t visits = 0;
-    while (true) {
+    // Every node of an acyclic tree is entered at most once per instance
+    // entry; the cap only bounds the loop should a malformed tree reach it.
+    uint32_t steps = 0;
+    while (++steps < kMaxTraversalSteps) {
         if (ref == kRefReturn) {
             in_blas = false;
             ao = o;

@@ -30,6 +30,8 @@
 #include <vector>
 
 #include "../../include/avrt.h"
+#include "avrt_ctx.h"
+#include "scene_convert.h"
 #include "avrt_device.h"
 #include "avrt_internal.h"
 
@@ -585,71 +587,20 @@ __global__ __launch_bounds__(kBlock) void k_display_only(FrameParams fp, const f
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
 } // namespace
-
-struct avrt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    AvrtCreateOptions opts{};
-    bool ref_order = false;
-
-    // geometry (host copies drive the per-mesh conversion)
-    std::vector<AvrtVertex> h_verts;
-    std::vector<uint32_t> h_indices;
-    std::vector<AvrtBVHNode> h_blas;
-    std::vector<Node2> h_blas2;
-    std::vector<TriPre> h_tris;
-    std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
-    std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase
-    bool geometry_loaded = false;
-    DevBuf d_verts, d_indices, d_blas2, d_tris;
-
-    // scene
-    std::vector<AvrtObject> h_objects;
-    std::vector<uint32_t> h_lights;
-    uint32_t nobj = 0, nlights = 0, nquad_lights = 0, ntlas2 = 0;
-    bool has_volume = false, has_tri_or_mesh = false;
-    bool scene_loaded = false;
-    DevBuf d_objects, d_geom, d_tlas2, d_lights;
-
-    // textures
-    DevBuf d_tex;
-    uint32_t tw = 0, th = 0, tl = 0;
-
-    // frame targets
-    DevBuf d_image, d_display;
-    uint32_t img_w = 0, img_h = 0;
-
-    // path buffers
-    DevBuf so[2], sd[2], st[2], hitA, hitB, Lpath, shp, shn, sha, shw, counters;
-    size_t cap_paths = 0;
-
-    AvrtStats stats{};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_a = nullptr, ev_b = nullptr;
-};
 
 static std::string g_create_error;
 
-static int set_err(avrt_ctx *c, int code, const std::string &msg)
+namespace avrt {
+
+int set_err(avrt_ctx *c, int code, const std::string &msg)
 {
     if (c) c->err = msg;
     else g_create_error = msg;
     return code;
 }
 
-#define HIPX(ctx, call)                                                                                  \
-    do {                                                                                                 \
-        hipError_t e_ = (call);                                                                          \
-        if (e_ != hipSuccess) return set_err((ctx), AVRT_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-static int ensure(avrt_ctx *c, DevBuf &b, size_t bytes)
+int ensure(avrt_ctx *c, DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return AVRT_OK;
@@ -664,133 +615,18 @@ static int ensure(avrt_ctx *c, DevBuf &b, size_t bytes)
     return AVRT_OK;
 }
 
-static void release(DevBuf &b)
+void release(DevBuf &b)
 {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
 }
 
-// --- conversion of reference BVH arrays into Node2 trees ----------------------
-namespace {
+} // namespace avrt
 
-struct Converter {
-    const AvrtBVHNode *nodes;
-    uint32_t base;      // index of the tree's root in `nodes`
-    uint32_t avail;     // nodes available from base
-    std::vector<Node2> *out;
-    std::string err;
-
-    static void set_child(Node2 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
-    {
-        n.bx[c] = lo[0];
-        n.bx[c + 2] = hi[0];
-        n.by[c] = lo[1];
-        n.by[c + 2] = hi[1];
-        n.bz[c] = lo[2];
-        n.bz[c + 2] = hi[2];
-        n.ref[c] = ref;
-    }
-    static void set_empty(Node2 &n, int c)
-    {
-        const float far_pt[3] = {1e30f, 1e30f, 1e30f};
-        set_child(n, c, far_pt, far_pt, kRefInvalid);
-    }
-
-    // Reference for a leaf [first, first+count) whose box is lo/hi. Leaves
-    // over 127 primitives become small subtrees of the same box.
-    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref)
-    {
-        if (count == 0) {
-            err = "empty leaf";
-            return false;
-        }
-        if (first + count > kMaxLeafFirst) {
-            err = "leaf slot beyond 2^24";
-            return false;
-        }
-        if (count <= kMaxLeafCount) {
-            *ref = make_leaf(first, count);
-            return true;
-        }
-        const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
-        const uint32_t half = count / 2;
-        uint32_t r0, r1;
-        if (!leaf_ref(first, half, lo, hi, &r0)) return false;
-        if (!leaf_ref(first + half, count - half, lo, hi, &r1)) return false;
-        set_child((*out)[me], 0, lo, hi, r0);
-        set_child((*out)[me], 1, lo, hi, r1);
-        *ref = me;
-        return true;
-    }
-
-    bool ref_of(uint32_t local, int depth, uint32_t *ref)
-    {
-        if (local >= avail || depth > 64) {
-            err = "BVH child index out of range";
-            return false;
-        }
-        const AvrtBVHNode &n = nodes[base + local];
-        if (n.count > 0) {
-            const float lo[3] = {n.min_x, n.min_y, n.min_z};
-            const float hi[3] = {n.max_x, n.max_y, n.max_z};
-            if (n.left < 0) {
-                err = "negative leaf slot";
-                return false;
-            }
-            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref);
-        }
-        if (n.left < 0 || n.right < 0) {
-            err = "negative child index";
-            return false;
-        }
-        const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
-        uint32_t r[2];
-        const uint32_t kids[2] = {static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.right)};
-        for (int c = 0; c < 2; c++) {
-            if (!ref_of(kids[c], depth + 1, &r[c])) return false;
-        }
-        for (int c = 0; c < 2; c++) {
-            const AvrtBVHNode &ch = nodes[base + kids[c]];
-            const float lo[3] = {ch.min_x, ch.min_y, ch.min_z};
-            const float hi[3] = {ch.max_x, ch.max_y, ch.max_z};
-            set_child((*out)[me], c, lo, hi, r[c]);
-        }
-        return true;
-    }
-
-    // Appends the tree rooted at nodes[base] behind a super-root; returns the
-    // super-root's index.
-    bool convert(uint32_t *super)
-    {
-        const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
-        uint32_t r;
-        if (!ref_of(0, 0, &r)) return false;
-        const AvrtBVHNode &root = nodes[base];
-        const float lo[3] = {root.min_x, root.min_y, root.min_z};
-        const float hi[3] = {root.max_x, root.max_y, root.max_z};
-        set_child((*out)[me], 0, lo, hi, r);
-        set_empty((*out)[me], 1);
-        *super = me;
-        return true;
-    }
-};
-
-inline void hcross(const float a[3], const float b[3], float o[3])
-{
-    o[0] = a[1] * b[2] - a[2] * b[1];
-    o[1] = a[2] * b[0] - a[0] * b[2];
-    o[2] = a[0] * b[1] - a[1] * b[0];
-}
-inline float hdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-
-} // namespace
 
 // Prepares BLAS trees and triangle records for every mesh an instance uses.
-static int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ninst, bool *changed)
+int avrt::prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ninst, bool *changed)
 {
     *changed = false;
     for (uint32_t i = 0; i < ninst; i++) {
@@ -801,9 +637,11 @@ static int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ni
             return set_err(c, AVRT_E_INVALID_ARG, "instance triangles beyond the uploaded index buffer");
         if (m.blasBase >= c->h_blas.size()) return set_err(c, AVRT_E_INVALID_ARG, "instance blasBase beyond the BLAS buffer");
         if (c->blas_root.find(m.blasBase) == c->blas_root.end()) {
-            Converter cv{c->h_blas.data(), m.blasBase, static_cast<uint32_t>(c->h_blas.size() - m.blasBase), &c->h_blas2, {}};
             uint32_t super = 0;
-            if (!cv.convert(&super)) return set_err(c, AVRT_E_INVALID_ARG, "BLAS at " + std::to_string(m.blasBase) + ": " + cv.err);
+            std::string why;
+            if (!convert_tree(c->h_blas.data(), m.blasBase, static_cast<uint32_t>(c->h_blas.size() - m.blasBase), c->h_blas2,
+                              &super, &why))
+                return set_err(c, AVRT_E_INVALID_ARG, "BLAS at " + std::to_string(m.blasBase) + ": " + why);
             c->blas_root[m.blasBase] = super;
             *changed = true;
         }
@@ -827,20 +665,7 @@ static int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ni
                 p[k][1] = v.py;
                 p[k][2] = v.pz;
             }
-            TriPre t{};
-            const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
-            const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
-            float n[3];
-            hcross(e1, e2, n);
-            const float nl = hdot(n, n);
-            for (int k = 0; k < 3; k++) {
-                t.p0[k] = p[0][k];
-                t.e1[k] = e1[k];
-                t.e2[k] = e2[k];
-                t.w[k] = n[k] / nl;
-            }
-            t.d0 = hdot(n, p[0]);
-            t.valid = (nl < 1e-12f) ? 0u : 1u;
+            const TriPre t = make_tripre(p[0], p[1], p[2]);
             c->h_tris[tbase + s] = t;
         }
         *changed = true;
@@ -848,13 +673,20 @@ static int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ni
     return AVRT_OK;
 }
 
-static int upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes)
+int avrt::upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes)
 {
     int rc = ensure(c, b, bytes);
     if (rc) return rc;
     if (bytes) HIPX(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
     return AVRT_OK;
+}
+
+int avrt::upload_mesh_tables(avrt_ctx *c)
+{
+    int rc = upload_vec(c, c->d_blas2, c->h_blas2.data(), sizeof(Node2) * c->h_blas2.size());
+    if (rc) return rc;
+    return upload_vec(c, c->d_tris, c->h_tris.data(), sizeof(TriPre) * c->h_tris.size());
 }
 
 static SceneDev scene_dev(const avrt_ctx *c)
@@ -871,69 +703,6 @@ static SceneDev scene_dev(const avrt_ctx *c)
     s.tex = static_cast<const uchar4 *>(c->d_tex.p);
     s.nobj = c->nobj;
     return s;
-}
-
-// Builds the per-slot geometry records the leaf tests read.
-static int build_geom(avrt_ctx *c, const AvrtObject *objects, uint32_t n, const AvrtMeshInstance *inst, uint32_t ninst,
-                      std::vector<ObjGeom> &geom)
-{
-    geom.assign(n, ObjGeom{});
-    c->has_volume = false;
-    c->has_tri_or_mesh = false;
-    for (uint32_t i = 0; i < n; i++) {
-        const AvrtObject &o = objects[i];
-        ObjGeom &g = geom[i];
-        if (o.colorType == AVRT_ISOTROPIC) c->has_volume = true;
-        uint32_t flags = o.shapeType & 0xFFu;
-        switch (o.shapeType) {
-        case AVRT_SHAPE_SPHERE:
-            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = o.radius;
-            g.b[0] = o.x2 - o.x; g.b[1] = o.y2 - o.y; g.b[2] = o.z2 - o.z;
-            flags |= kGeomValid;
-            break;
-        case AVRT_SHAPE_QUAD:
-        case AVRT_SHAPE_TRIANGLE: {
-            c->has_tri_or_mesh |= (o.shapeType == AVRT_SHAPE_TRIANGLE);
-            const float u[3] = {o.u_x, o.u_y, o.u_z}, v[3] = {o.v_x, o.v_y, o.v_z}, P[3] = {o.x, o.y, o.z};
-            float nn[3];
-            hcross(u, v, nn);
-            const float nl = hdot(nn, nn);
-            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = hdot(nn, P);
-            for (int k = 0; k < 3; k++) {
-                g.b[k] = u[k];
-                g.c[k] = v[k];
-                const float w = nn[k] / nl;
-                std::memcpy(&g.d[k], &w, 4);
-            }
-            if (!(nl < 1e-8f)) flags |= kGeomValid;
-            break;
-        }
-        case AVRT_SHAPE_MESH: {
-            c->has_tri_or_mesh = true;
-            if (o.instanceIndex >= ninst) return set_err(c, AVRT_E_INVALID_ARG, "mesh object instanceIndex out of range");
-            const AvrtMeshInstance &m = inst[o.instanceIndex];
-            g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = (m.scale == 0.0f) ? 1.0f : m.scale;
-            g.b[0] = m.qx; g.b[1] = m.qy; g.b[2] = m.qz; g.b[3] = m.qw;
-            if (m.triangleCount > 0) {
-                auto it = c->blas_root.find(m.blasBase);
-                if (it == c->blas_root.end()) return set_err(c, AVRT_E_STATE, "mesh BLAS not prepared");
-                const uint32_t words[4] = {it->second, m.indexBase / 3, m.vertexBase, m.indexBase};
-                std::memcpy(g.c, words, sizeof(words));
-                flags |= kGeomValid;
-            }
-            break;
-        }
-        default: {
-            const float s = std::sin(o.rotation), cs = std::cos(o.rotation);
-            g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = s;
-            g.b[0] = o.half_x; g.b[1] = o.half_y; g.b[2] = o.half_z; g.b[3] = cs;
-            flags |= kGeomValid;
-            break;
-        }
-        }
-        g.d[3] = flags;
-    }
-    return AVRT_OK;
 }
 
 extern "C" {
@@ -979,9 +748,11 @@ int avrt_destroy(avrt_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects, &c->d_geom, &c->d_tlas2,
-                      &c->d_lights, &c->d_tex, &c->d_image, &c->d_display, &c->so[0], &c->so[1], &c->sd[0],
-                      &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn, &c->sha,
-                      &c->shw, &c->counters};
+                      &c->d_lights, &c->d_instances, &c->d_tex, &c->d_image, &c->d_display, &c->so[0], &c->so[1],
+                      &c->sd[0], &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn,
+                      &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
+                      &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
+                      &c->rb.scratch, &c->rb.info};
     for (DevBuf *b : bufs) release(*b);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
         if (e) (void)hipEventDestroy(e);
@@ -1040,20 +811,19 @@ int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_co
         if (rc) return rc;
     }
     if (changed) {
-        int rc = upload_vec(c, c->d_blas2, c->h_blas2.data(), sizeof(Node2) * c->h_blas2.size());
-        if (rc) return rc;
-        rc = upload_vec(c, c->d_tris, c->h_tris.data(), sizeof(TriPre) * c->h_tris.size());
+        int rc = upload_mesh_tables(c);
         if (rc) return rc;
     }
     std::vector<ObjGeom> geom;
-    int rc = build_geom(c, objects, object_count, instances, instance_count, geom);
-    if (rc) return rc;
+    std::string why;
+    int rc = build_geom_records(objects, object_count, instances, instance_count, c->blas_root, geom, &c->has_volume,
+                                &c->has_tri_or_mesh, &why);
+    if (rc) return set_err(c, rc, why);
     if (c->has_volume) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
     std::vector<Node2> t2;
     if (object_count) {
-        Converter cv{tlas, 0, node_count, &t2, {}};
         uint32_t super = 0;
-        if (!cv.convert(&super)) return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + cv.err);
+        if (!convert_tree(tlas, 0, node_count, t2, &super, &why)) return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + why);
         // leaf slots must address uploaded objects
         for (uint32_t i = 0; i < node_count; i++)
             if (tlas[i].count > 0 && static_cast<uint64_t>(tlas[i].left) + tlas[i].count > object_count)
@@ -1067,8 +837,9 @@ int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_co
     if (rc) return rc;
     rc = upload_vec(c, c->d_lights, light_ids, sizeof(uint32_t) * light_count);
     if (rc) return rc;
-    c->h_objects.assign(objects, objects + object_count);
-    c->h_lights.assign(light_ids, light_ids + light_count);
+    rc = upload_vec(c, c->d_instances, instances, sizeof(AvrtMeshInstance) * instance_count);
+    if (rc) return rc;
+    c->rb.ready = false; // an explicit upload replaces a device-built scene
     c->nobj = object_count;
     c->ntlas2 = static_cast<uint32_t>(t2.size());
     c->nlights = light_count;
@@ -1098,6 +869,13 @@ int avrt_upload_textures(avrt_ctx *c, const uint8_t *rgba8_srgb, uint32_t width,
 }
 
 } // extern "C"
+
+static inline void hcross(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
 
 // Camera basis on the host (ray_trace.comp.hybrid.hlsl:1347-1359).
 static void camera_params(const AvrtConfig &cfg, FrameParams &fp)
@@ -1352,6 +1130,7 @@ int avrt_get_stats(const avrt_ctx *c, AvrtStats *out)
 {
     if (c == nullptr || out == nullptr) return AVRT_E_INVALID_ARG;
     *out = c->stats;
+    out->ms_scene_step = c->rb.last_ms;
     return AVRT_OK;
 }
 
@@ -1418,22 +1197,3 @@ int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object,
 
 } // extern "C"
 
-// Device-side scene rebuild entry points (implemented in a later step).
-extern "C" {
-int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *, uint32_t, const AvrtMeshInstance *, uint32_t,
-                             const AvrtAABB *)
-{
-    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
-}
-int avrt_scene_step(avrt_ctx *c, const float *, float) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
-int avrt_scene_read(avrt_ctx *c, AvrtBVHNode *, uint32_t *, AvrtObject *, AvrtMeshInstance *, uint32_t *)
-{
-    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
-}
-int avrt_refit_blas(avrt_ctx *c) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
-int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *, uint32_t, uint32_t)
-{
-    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
-}
-int avrt_read_blas(avrt_ctx *c, AvrtBVHNode *, uint32_t) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
-}

@@ -1,0 +1,942 @@
+// avrt_scene.hip - per-frame scene rebuild on the device.
+//
+// RebuildAcceleration (renderer/renderer.cpp:1087-1130) runs on the CPU every
+// dirty frame in the reference: GatherRenderables + the *Bounds functions
+// (scene/shapes.h:71-145,379-426), BuildBVH (scene/bvh.h:275-295), the
+// permutation applied to the objects, PackObjects (renderer.cpp:815-929) and
+// LightIndices (:803-813). Here the same steps run as device kernels on the
+// context's stream, with the rigid-body motion of the instances first:
+//
+//   k_integrate   position += v*dt; q += dt/2 (w x q), normalised
+//   k_bounds      world AABB per renderable (sphere, quad, triangle, mesh)
+//   k_build_tlas  one 1024-thread workgroup builds the binned-SAH BVH level by
+//                 level: segmented box / centroid reductions with atomics,
+//                 12-bin histograms per axis, the SAH sweeps, and the
+//                 partition of libstdc++'s two-ended std::partition restated
+//                 as two prefix scans (the k-th misplaced element from the
+//                 left swaps with the k-th misplaced one from the right). Node
+//                 numbering is converted to DFS preorder at the end from the
+//                 subtree sizes, so the arrays equal avrt_build_bvh's.
+//   k_pack        objects, instances and light slots in slot order
+//   k_convert     TLAS into the renderer's Node2 layout and ObjGeom records
+//
+// Where the reference folds boxes with std::min/std::max, equal values keep
+// the first operand. The node-box reductions reproduce that (the bits of the
+// first primitive in index order win among equal values, which matters only
+// for the sign of zero) by reducing 64-bit keys (value, position).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+#include "avrt_ctx.h"
+#include "avrt_device.h"
+#include "avrt_internal.h"
+
+using namespace avrt;
+
+namespace {
+
+constexpr int kBT = 1024; // build workgroup size
+constexpr uint32_t kErrBox = 1u;
+constexpr uint32_t kErrLeaf = 2u;
+constexpr uint32_t kErrBuild = 4u;
+
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; } // std::min
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; } // std::max
+
+__device__ __forceinline__ uint32_t f2sort(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float sort2f(uint32_t s)
+{
+    return __uint_as_float((s & 0x80000000u) ? (s & 0x7FFFFFFFu) : ~s);
+}
+__device__ __forceinline__ uint32_t canon_sort(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) == 0u) u = 0u;
+    return f2sort(__uint_as_float(u));
+}
+__device__ __forceinline__ unsigned long long key_min(float v, uint32_t pos)
+{
+    return (static_cast<unsigned long long>(canon_sort(v)) << 32) | pos;
+}
+__device__ __forceinline__ unsigned long long key_max(float v, uint32_t pos)
+{
+    return (static_cast<unsigned long long>(canon_sort(v)) << 32) | (0xFFFFFFFFu - pos);
+}
+
+__device__ __forceinline__ float comp(const AvrtAABB &b, int k)
+{
+    switch (k) {
+    case 0: return b.min_x;
+    case 1: return b.min_y;
+    case 2: return b.min_z;
+    case 3: return b.max_x;
+    case 4: return b.max_y;
+    default: return b.max_z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-body step and bounds
+// ---------------------------------------------------------------------------
+__global__ void k_integrate(uint32_t m, float dt, const float *vel, AvrtMeshInstance *inst)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    AvrtMeshInstance q = inst[i];
+    const float *v = vel + 6 * i;
+    q.px = q.px + v[0] * dt;
+    q.py = q.py + v[1] * dt;
+    q.pz = q.pz + v[2] * dt;
+    const float wx = v[3], wy = v[4], wz = v[5];
+    // dq/dt = 0.5 * (w, 0) (x) q
+    const float dx = wx * q.qw + wy * q.qz - wz * q.qy;
+    const float dy = -wx * q.qz + wy * q.qw + wz * q.qx;
+    const float dz = wx * q.qy - wy * q.qx + wz * q.qw;
+    const float dw = -wx * q.qx - wy * q.qy - wz * q.qz;
+    const float h = 0.5f * dt;
+    float x = q.qx + h * dx, y = q.qy + h * dy, z = q.qz + h * dz, w = q.qw + h * dw;
+    const float inv = 1.0f / sqrtf(x * x + y * y + z * z + w * w);
+    q.qx = x * inv;
+    q.qy = y * inv;
+    q.qz = z * inv;
+    q.qw = w * inv;
+    inst[i] = q;
+}
+
+// Keeps the mesh objects' positions in step with their instances
+// (MeshToGPU writes the transform position, shapes.h:270-274); the previous
+// position goes to x2 (ApplyPreviousTransforms, renderer.cpp:1034-1085).
+__global__ void k_sync_mesh_objects(uint32_t n, const int32_t *mesh_rank, const AvrtMeshInstance *inst,
+                                    AvrtObject *objects)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r = mesh_rank[i];
+    if (r < 0) return;
+    AvrtObject o = objects[i];
+    const AvrtMeshInstance q = inst[r];
+    o.x2 = o.x;
+    o.y2 = o.y;
+    o.z2 = o.z;
+    o.x = q.px;
+    o.y = q.py;
+    o.z = q.pz;
+    objects[i] = o;
+}
+
+__global__ void k_bounds(uint32_t n, const AvrtObject *objects, const int32_t *mesh_rank, const AvrtMeshInstance *inst,
+                         const AvrtAABB *local, AvrtAABB *bounds, uint32_t *info)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const AvrtObject o = objects[i];
+    AvrtAABB b;
+    if (o.shapeType == AVRT_SHAPE_SPHERE) {
+        const float r = o.radius;
+        b = AvrtAABB{o.x - r, o.y - r, o.z - r, o.x + r, o.y + r, o.z + r};
+    } else if (o.shapeType == AVRT_SHAPE_QUAD) {
+        // QuadBounds: Surround(Surround2Points(P, P+u+v), Surround2Points(P+u, P+v))
+        const float ax = o.x, ay = o.y, az = o.z;
+        const float bx = o.x + o.u_x + o.v_x, by = o.y + o.u_y + o.v_y, bz = o.z + o.u_z + o.v_z;
+        const float cx = o.x + o.u_x, cy = o.y + o.u_y, cz = o.z + o.u_z;
+        const float dx = o.x + o.v_x, dy = o.y + o.v_y, dz = o.z + o.v_z;
+        b = AvrtAABB{smin(smin(ax, bx), smin(cx, dx)), smin(smin(ay, by), smin(cy, dy)), smin(smin(az, bz), smin(cz, dz)),
+                     smax(smax(ax, bx), smax(cx, dx)), smax(smax(ay, by), smax(cy, dy)), smax(smax(az, bz), smax(cz, dz))};
+    } else if (o.shapeType == AVRT_SHAPE_TRIANGLE) {
+        // TriangleBounds: Surround2Points(P, P+u) then SurroundPoint(P+v)
+        const float ux = o.x + o.u_x, uy = o.y + o.u_y, uz = o.z + o.u_z;
+        const float vx = o.x + o.v_x, vy = o.y + o.v_y, vz = o.z + o.v_z;
+        b = AvrtAABB{smin(smin(o.x, ux), vx), smin(smin(o.y, uy), vy), smin(smin(o.z, uz), vz),
+                     smax(smax(o.x, ux), vx), smax(smax(o.y, uy), vy), smax(smax(o.z, uz), vz)};
+    } else if (o.shapeType == AVRT_SHAPE_MESH) {
+        // MeshBounds: eight corners of the local box, rotated, translated.
+        const AvrtMeshInstance q = inst[mesh_rank[i]];
+        const AvrtAABB l = local[mesh_rank[i]];
+        b = AvrtAABB{FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (!(l.min_x > l.max_x)) {
+            const float s = q.scale == 0.0f ? 1.0f : q.scale;
+            for (int c = 0; c < 8; c++) {
+                dev::F3 p = dev::f3((c & 1) ? l.max_x : l.min_x, (c & 2) ? l.max_y : l.min_y, (c & 4) ? l.max_z : l.min_z);
+                if (s != 1.0f) p = dev::f3(p.x * s, p.y * s, p.z * s);
+                const dev::F3 r = dev::quat_rotate(q.qx, q.qy, q.qz, q.qw, p);
+                const float wx = q.px + r.x, wy = q.py + r.y, wz = q.pz + r.z;
+                b.min_x = smin(b.min_x, wx);
+                b.min_y = smin(b.min_y, wy);
+                b.min_z = smin(b.min_z, wz);
+                b.max_x = smax(b.max_x, wx);
+                b.max_y = smax(b.max_y, wy);
+                b.max_z = smax(b.max_z, wz);
+            }
+        }
+    } else {
+        b = AvrtAABB{0, 0, 0, 0, 0, 0};
+        atomicOr(&info[1], kErrBox);
+    }
+    bounds[i] = b;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup-wide exclusive scan of n values (chunks of kBT), out[n] = total.
+// ---------------------------------------------------------------------------
+__device__ void wg_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *sh_wave, uint32_t *sh_carry)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) *sh_carry = 0u;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBT) {
+        const uint32_t i = base + tid;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t x = v;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63u) sh_wave[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t w = lane < (kBT / 64) ? sh_wave[lane] : 0u;
+            uint32_t acc = w;
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(acc, off);
+                if (lane >= off) acc += y;
+            }
+            if (lane < (kBT / 64)) sh_wave[lane] = acc - w;
+        }
+        __syncthreads();
+        const uint32_t excl = x - v + sh_wave[wave] + *sh_carry;
+        if (i < n) out[i] = excl;
+        __syncthreads();
+        if (tid == kBT - 1) *sh_carry = excl + v;
+        __syncthreads();
+    }
+    if (tid == 0) out[n] = *sh_carry;
+    __syncthreads();
+}
+
+struct BuildWork {
+    uint32_t *idx, *idx2, *flag, *flag2, *scanA, *scanB, *posL, *posR, *fbeg;
+    float *cen;
+    uint32_t *t_start, *t_end, *t_depth, *t_left, *t_right, *t_size, *t_pre, *t_leaf, *t_mid, *t_part, *t_kids;
+    int32_t *t_axis, *t_split;
+    unsigned long long *t_key; // 6 per task
+    float *t_box;              // 6 per task
+    uint32_t *t_cbox;          // 6 per task
+    float *t_scale;            // 3 per task
+    uint32_t *bins;            // level-local: [L][3][12][7]
+};
+
+__device__ __forceinline__ uint32_t find_task(const uint32_t *fbeg, uint32_t L, uint32_t f)
+{
+    uint32_t lo = 0, hi = L; // largest k with fbeg[k] <= f
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (fbeg[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float half_area(float lx, float ly, float lz, float hx, float hy, float hz)
+{
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ int sah_bin(float c, float cmin, float scale)
+{
+    const int b = static_cast<int>((c - cmin) * scale);
+    return b < 11 ? b : 11;
+}
+
+__global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *bounds, BuildWork w,
+                                                    AvrtBVHNode *out, uint32_t *order, uint32_t *info)
+{
+    __shared__ uint32_t sh_wave[kBT / 64];
+    __shared__ uint32_t sh_carry;
+    __shared__ uint32_t sh_L, sh_lb, sh_ntasks, sh_nlev;
+    __shared__ uint32_t sh_levels[64];
+    const uint32_t tid = threadIdx.x;
+    if (N == 0) {
+        if (tid == 0) info[0] = 0;
+        return;
+    }
+    for (uint32_t i = tid; i < N; i += kBT) {
+        w.idx[i] = i;
+        const AvrtAABB b = bounds[i];
+        w.cen[3 * i + 0] = (b.min_x + b.max_x) * 0.5f;
+        w.cen[3 * i + 1] = (b.min_y + b.max_y) * 0.5f;
+        w.cen[3 * i + 2] = (b.min_z + b.max_z) * 0.5f;
+    }
+    if (tid == 0) {
+        w.t_start[0] = 0;
+        w.t_end[0] = N;
+        w.t_depth[0] = 0;
+        sh_L = 1;
+        sh_lb = 0;
+        sh_ntasks = 1;
+        sh_nlev = 0;
+    }
+    __syncthreads();
+    while (true) {
+        const uint32_t L = sh_L, lb = sh_lb;
+        if (L == 0) break;
+        if (tid == 0) sh_levels[sh_nlev] = lb;
+        // flat index space of this level's positions
+        for (uint32_t k = tid; k < L; k += kBT) w.flag[k] = w.t_end[lb + k] - w.t_start[lb + k];
+        __syncthreads();
+        wg_scan(w.flag, L, w.fbeg, sh_wave, &sh_carry);
+        const uint32_t F = w.fbeg[L];
+        for (uint32_t k = tid; k < L; k += kBT) {
+            const uint32_t t = lb + k;
+            for (int c = 0; c < 3; c++) {
+                w.t_key[6 * t + c] = ~0ull;
+                w.t_key[6 * t + 3 + c] = 0ull;
+                w.t_cbox[6 * t + c] = f2sort(FLT_MAX);
+                w.t_cbox[6 * t + 3 + c] = f2sort(-FLT_MAX);
+            }
+        }
+        __syncthreads();
+        // node box (first-in-index-order among equal values) and centroid box
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            const uint32_t prim = w.idx[pos];
+            const AvrtAABB b = bounds[prim];
+            atomicMin(&w.t_key[6 * t + 0], key_min(b.min_x, pos));
+            atomicMin(&w.t_key[6 * t + 1], key_min(b.min_y, pos));
+            atomicMin(&w.t_key[6 * t + 2], key_min(b.min_z, pos));
+            atomicMax(&w.t_key[6 * t + 3], key_max(b.max_x, pos));
+            atomicMax(&w.t_key[6 * t + 4], key_max(b.max_y, pos));
+            atomicMax(&w.t_key[6 * t + 5], key_max(b.max_z, pos));
+            for (int c = 0; c < 3; c++) {
+                const uint32_t s = f2sort(w.cen[3 * prim + c]);
+                atomicMin(&w.t_cbox[6 * t + c], s);
+                atomicMax(&w.t_cbox[6 * t + 3 + c], s);
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < L; k += kBT) {
+            const uint32_t t = lb + k;
+            for (int c = 0; c < 6; c++) {
+                const uint32_t low = static_cast<uint32_t>(w.t_key[6 * t + c]);
+                const uint32_t pos = c < 3 ? low : 0xFFFFFFFFu - low;
+                w.t_box[6 * t + c] = comp(bounds[w.idx[pos]], c);
+            }
+            const uint32_t count = w.t_end[t] - w.t_start[t];
+            const bool leaf = count <= 4 || w.t_depth[t] >= 20;
+            w.t_leaf[t] = leaf ? 1u : 0u;
+            if (!leaf) {
+                for (int a = 0; a < 3; a++) {
+                    const float extent = sort2f(w.t_cbox[6 * t + 3 + a]) - sort2f(w.t_cbox[6 * t + a]);
+                    w.t_scale[3 * t + a] = (extent <= 0.0f) ? 0.0f : 12.0f / extent;
+                    uint32_t *bn = w.bins + (static_cast<size_t>(k) * 3 + a) * 12 * 7;
+                    for (int b = 0; b < 12; b++) {
+                        for (int c = 0; c < 3; c++) {
+                            bn[b * 7 + c] = f2sort(FLT_MAX);
+                            bn[b * 7 + 3 + c] = f2sort(-FLT_MAX);
+                        }
+                        bn[b * 7 + 6] = 0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // 12-bin histograms per axis
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            if (w.t_leaf[t]) continue;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            const uint32_t prim = w.idx[pos];
+            const AvrtAABB b = bounds[prim];
+            for (int a = 0; a < 3; a++) {
+                const float scale = w.t_scale[3 * t + a];
+                if (scale == 0.0f) continue;
+                const int bin = sah_bin(w.cen[3 * prim + a], sort2f(w.t_cbox[6 * t + a]), scale);
+                uint32_t *bn = w.bins + ((static_cast<size_t>(k) * 3 + a) * 12 + bin) * 7;
+                atomicMin(&bn[0], f2sort(b.min_x));
+                atomicMin(&bn[1], f2sort(b.min_y));
+                atomicMin(&bn[2], f2sort(b.min_z));
+                atomicMax(&bn[3], f2sort(b.max_x));
+                atomicMax(&bn[4], f2sort(b.max_y));
+                atomicMax(&bn[5], f2sort(b.max_z));
+                atomicAdd(&bn[6], 1u);
+            }
+        }
+        __syncthreads();
+        // SAH sweeps (bvh.h:177-235)
+        for (uint32_t k = tid; k < L; k += kBT) {
+            const uint32_t t = lb + k;
+            if (w.t_leaf[t]) continue;
+            int best_axis = -1, best_split = 0;
+            float best_cost = FLT_MAX;
+            for (int a = 0; a < 3; a++) {
+                if (w.t_scale[3 * t + a] == 0.0f) continue;
+                const uint32_t *bn = w.bins + (static_cast<size_t>(k) * 3 + a) * 12 * 7;
+                float lc[11];
+                int ln[11];
+                float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                int n = 0;
+                for (int s = 0; s < 11; s++) {
+                    for (int c = 0; c < 3; c++) {
+                        lo[c] = smin(lo[c], sort2f(bn[s * 7 + c]));
+                        hi[c] = smax(hi[c], sort2f(bn[s * 7 + 3 + c]));
+                    }
+                    n += static_cast<int>(bn[s * 7 + 6]);
+                    ln[s] = n;
+                    lc[s] = n > 0 ? half_area(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]) * static_cast<float>(n) : 0.0f;
+                }
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = FLT_MAX;
+                    hi[c] = -FLT_MAX;
+                }
+                n = 0;
+                for (int s = 11; s > 0; s--) {
+                    for (int c = 0; c < 3; c++) {
+                        lo[c] = smin(lo[c], sort2f(bn[s * 7 + c]));
+                        hi[c] = smax(hi[c], sort2f(bn[s * 7 + 3 + c]));
+                    }
+                    n += static_cast<int>(bn[s * 7 + 6]);
+                    if (ln[s - 1] == 0 || n == 0) continue;
+                    const float cost = lc[s - 1] + half_area(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]) * static_cast<float>(n);
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = a;
+                        best_split = s;
+                    }
+                }
+            }
+            w.t_axis[t] = best_axis;
+            w.t_split[t] = best_split;
+        }
+        __syncthreads();
+        // predicate of std::partition per position
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            uint32_t p = 0;
+            if (!w.t_leaf[t] && w.t_axis[t] >= 0) {
+                const int a = w.t_axis[t];
+                const uint32_t prim = w.idx[w.t_start[t] + (f - w.fbeg[k])];
+                p = sah_bin(w.cen[3 * prim + a], sort2f(w.t_cbox[6 * t + a]), w.t_scale[3 * t + a]) < w.t_split[t] ? 1u : 0u;
+            }
+            w.flag2[f] = p;
+        }
+        __syncthreads();
+        wg_scan(w.flag2, F, w.scanA, sh_wave, &sh_carry);
+        for (uint32_t k = tid; k < L; k += kBT) {
+            const uint32_t t = lb + k;
+            const uint32_t start = w.t_start[t], end = w.t_end[t], count = end - start;
+            uint32_t mid = start + count / 2;
+            uint32_t part = 0;
+            if (!w.t_leaf[t] && w.t_axis[t] >= 0) {
+                const uint32_t T = w.scanA[w.fbeg[k + 1]] - w.scanA[w.fbeg[k]];
+                const uint32_t m = start + T;
+                if (m > start && m < end) {
+                    mid = m;
+                    part = 1;
+                }
+            }
+            w.t_mid[t] = mid;
+            w.t_part[t] = part;
+        }
+        __syncthreads();
+        // misplaced on the left (pred false below mid) and on the right (pred
+        // true at or above mid)
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            const uint32_t pr = w.flag2[f];
+            const bool on = w.t_part[t] != 0;
+            w.flag[f] = (on && pos < w.t_mid[t] && !pr) ? 1u : 0u;
+            w.flag2[f] = (on && pos >= w.t_mid[t] && pr) ? 1u : 0u;
+        }
+        __syncthreads();
+        wg_scan(w.flag, F, w.scanA, sh_wave, &sh_carry);
+        wg_scan(w.flag2, F, w.scanB, sh_wave, &sh_carry);
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            const uint32_t base = w.fbeg[k];
+            if (w.flag[f]) w.posL[base + (w.scanA[f] - w.scanA[base])] = pos;
+            if (w.flag2[f]) {
+                const uint32_t mr = w.scanB[w.fbeg[k + 1]] - w.scanB[base];
+                w.posR[base + (mr - 1u - (w.scanB[f] - w.scanB[base]))] = pos;
+            }
+        }
+        __syncthreads();
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            const uint32_t base = w.fbeg[k];
+            const uint32_t v = w.idx[pos];
+            uint32_t dst = pos;
+            if (w.flag[f]) {
+                dst = w.posR[base + (w.scanA[f] - w.scanA[base])];
+            } else if (w.flag2[f]) {
+                const uint32_t mr = w.scanB[w.fbeg[k + 1]] - w.scanB[base];
+                dst = w.posL[base + (mr - 1u - (w.scanB[f] - w.scanB[base]))];
+            }
+            // a swap partner always lies inside the same node's range
+            if (dst < w.t_start[t] || dst >= w.t_end[t]) {
+                atomicOr(&info[1], kErrBuild);
+                dst = pos;
+            }
+            w.idx2[dst] = v;
+        }
+        __syncthreads();
+        for (uint32_t f = tid; f < F; f += kBT) {
+            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+            w.idx[pos] = w.idx2[pos];
+        }
+        // children of this level, in order
+        for (uint32_t k = tid; k < L; k += kBT) w.flag[k] = w.t_leaf[lb + k] ? 0u : 2u;
+        __syncthreads();
+        wg_scan(w.flag, L, w.scanA, sh_wave, &sh_carry);
+        const uint32_t nkids = w.scanA[L];
+        const uint32_t next_lb = lb + L;
+        for (uint32_t k = tid; k < L; k += kBT) {
+            const uint32_t t = lb + k;
+            if (w.t_leaf[t]) continue;
+            const uint32_t c = next_lb + w.scanA[k];
+            w.t_left[t] = c;
+            w.t_right[t] = c + 1;
+            w.t_start[c] = w.t_start[t];
+            w.t_end[c] = w.t_mid[t];
+            w.t_depth[c] = w.t_depth[t] + 1;
+            w.t_start[c + 1] = w.t_mid[t];
+            w.t_end[c + 1] = w.t_end[t];
+            w.t_depth[c + 1] = w.t_depth[t] + 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            sh_lb = next_lb;
+            sh_L = nkids;
+            sh_ntasks = next_lb + nkids;
+            sh_nlev = sh_nlev + 1;
+        }
+        __syncthreads();
+    }
+    const uint32_t ntasks = sh_ntasks, nlev = sh_nlev;
+    // subtree sizes bottom-up, preorder numbers top-down
+    for (int l = static_cast<int>(nlev) - 1; l >= 0; l--) {
+        const uint32_t b = sh_levels[l], e = (l + 1 < static_cast<int>(nlev)) ? sh_levels[l + 1] : ntasks;
+        for (uint32_t t = b + tid; t < e; t += kBT)
+            w.t_size[t] = w.t_leaf[t] ? 1u : 1u + w.t_size[w.t_left[t]] + w.t_size[w.t_right[t]];
+        __syncthreads();
+    }
+    if (tid == 0) w.t_pre[0] = 0;
+    __syncthreads();
+    for (uint32_t l = 0; l < nlev; l++) {
+        const uint32_t b = sh_levels[l], e = (l + 1 < nlev) ? sh_levels[l + 1] : ntasks;
+        for (uint32_t t = b + tid; t < e; t += kBT) {
+            if (w.t_leaf[t]) continue;
+            w.t_pre[w.t_left[t]] = w.t_pre[t] + 1;
+            w.t_pre[w.t_right[t]] = w.t_pre[t] + 1 + w.t_size[w.t_left[t]];
+        }
+        __syncthreads();
+    }
+    const float thin = 0.0001f;
+    for (uint32_t t = tid; t < ntasks; t += kBT) {
+        AvrtBVHNode n;
+        float lo[3], hi[3];
+        for (int c = 0; c < 3; c++) {
+            lo[c] = w.t_box[6 * t + c];
+            hi[c] = w.t_box[6 * t + 3 + c];
+            if (hi[c] - lo[c] < thin) {
+                lo[c] -= thin * 0.5f;
+                hi[c] += thin * 0.5f;
+            }
+        }
+        n.min_x = lo[0];
+        n.min_y = lo[1];
+        n.min_z = lo[2];
+        n.pad0 = 0.0f;
+        n.max_x = hi[0];
+        n.max_y = hi[1];
+        n.max_z = hi[2];
+        const bool leaf = w.t_leaf[t] != 0;
+        n.left = leaf ? static_cast<int32_t>(w.t_start[t]) : static_cast<int32_t>(w.t_pre[w.t_left[t]]);
+        n.right = leaf ? -1 : static_cast<int32_t>(w.t_pre[w.t_right[t]]);
+        n.count = leaf ? static_cast<int32_t>(w.t_end[t] - w.t_start[t]) : 0;
+        n.padding = 0;
+        n.pad_trailing = 0;
+        out[w.t_pre[t]] = n;
+    }
+    for (uint32_t i = tid; i < N; i += kBT) order[i] = w.idx[i];
+    if (tid == 0) info[0] = ntasks;
+}
+
+// ---------------------------------------------------------------------------
+// Slot-order packing (PackObjects + LightIndices) and per-slot ObjGeom.
+// ---------------------------------------------------------------------------
+__device__ ObjGeom make_geom(const AvrtObject &o, const AvrtMeshInstance *inst, const uint4 *geomc, int32_t rank)
+{
+    ObjGeom g;
+    for (int k = 0; k < 4; k++) {
+        g.a[k] = 0.0f;
+        g.b[k] = 0.0f;
+        g.c[k] = 0.0f;
+        g.d[k] = 0u;
+    }
+    uint32_t flags = o.shapeType & 0xFFu;
+    if (o.shapeType == AVRT_SHAPE_SPHERE) {
+        g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = o.radius;
+        g.b[0] = o.x2 - o.x; g.b[1] = o.y2 - o.y; g.b[2] = o.z2 - o.z;
+        flags |= kGeomValid;
+    } else if (o.shapeType == AVRT_SHAPE_QUAD || o.shapeType == AVRT_SHAPE_TRIANGLE) {
+        const dev::F3 u = dev::f3(o.u_x, o.u_y, o.u_z), v = dev::f3(o.v_x, o.v_y, o.v_z);
+        const dev::F3 n = dev::cross(u, v);
+        const float nl = dev::dot(n, n);
+        g.a[0] = o.x; g.a[1] = o.y; g.a[2] = o.z; g.a[3] = dev::dot(n, dev::f3(o.x, o.y, o.z));
+        g.b[0] = u.x; g.b[1] = u.y; g.b[2] = u.z;
+        g.c[0] = v.x; g.c[1] = v.y; g.c[2] = v.z;
+        g.d[0] = __float_as_uint(n.x / nl);
+        g.d[1] = __float_as_uint(n.y / nl);
+        g.d[2] = __float_as_uint(n.z / nl);
+        if (!(nl < 1e-8f)) flags |= kGeomValid;
+    } else if (o.shapeType == AVRT_SHAPE_MESH) {
+        const AvrtMeshInstance m = inst[rank];
+        g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = (m.scale == 0.0f) ? 1.0f : m.scale;
+        g.b[0] = m.qx; g.b[1] = m.qy; g.b[2] = m.qz; g.b[3] = m.qw;
+        const uint4 c = geomc[rank];
+        g.c[0] = __uint_as_float(c.x); g.c[1] = __uint_as_float(c.y);
+        g.c[2] = __uint_as_float(c.z); g.c[3] = __uint_as_float(c.w);
+        if (m.triangleCount > 0) flags |= kGeomValid;
+    }
+    g.d[3] = flags;
+    return g;
+}
+
+__global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order, const AvrtObject *auth,
+                                              const int32_t *mesh_rank, const AvrtMeshInstance *auth_inst,
+                                              const uint4 *geomc, AvrtObject *objects, AvrtMeshInstance *instances,
+                                              ObjGeom *geom, uint32_t *lights)
+{
+    __shared__ uint32_t sh_wm[kBT / 64], sh_wl[kBT / 64];
+    __shared__ uint32_t sh_cm, sh_cl;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) {
+        sh_cm = 0;
+        sh_cl = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBT) {
+        const uint32_t s = base + tid;
+        const bool valid = s < n;
+        const uint32_t a = valid ? order[s] : 0u;
+        AvrtObject o = auth[a];
+        const uint32_t im = (valid && o.shapeType == AVRT_SHAPE_MESH) ? 1u : 0u;
+        const uint32_t il = (valid && o.colorType == AVRT_DIFFUSE_LIGHT) ? 1u : 0u;
+        uint32_t xm = im, xl = il;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t ym = __shfl_up(xm, off), yl = __shfl_up(xl, off);
+            if (lane >= off) {
+                xm += ym;
+                xl += yl;
+            }
+        }
+        if (lane == 63u) {
+            sh_wm[wave] = xm;
+            sh_wl[wave] = xl;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t am = 0, al = 0;
+            for (uint32_t k = 0; k < kBT / 64; k++) {
+                const uint32_t vm = sh_wm[k], vl = sh_wl[k];
+                sh_wm[k] = am;
+                sh_wl[k] = al;
+                am += vm;
+                al += vl;
+            }
+        }
+        __syncthreads();
+        const uint32_t rm = sh_cm + sh_wm[wave] + xm - im;
+        const uint32_t rl = sh_cl + sh_wl[wave] + xl - il;
+        if (valid) {
+            const int32_t rank = mesh_rank[a];
+            if (im) {
+                o.instanceIndex = rm;
+                instances[rm] = auth_inst[rank];
+            }
+            objects[s] = o;
+            geom[s] = make_geom(o, auth_inst, geomc, rank);
+            if (il) lights[rl] = s;
+        }
+        __syncthreads();
+        if (tid == kBT - 1) {
+            sh_cm = rm + im;
+            sh_cl = rl + il;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void set_child(Node2 &n, int c, const AvrtBVHNode &b, uint32_t ref)
+{
+    n.bx[c] = b.min_x;
+    n.bx[c + 2] = b.max_x;
+    n.by[c] = b.min_y;
+    n.by[c + 2] = b.max_y;
+    n.bz[c] = b.min_z;
+    n.bz[c + 2] = b.max_z;
+    n.ref[c] = ref;
+}
+
+__device__ __forceinline__ uint32_t ref_of(const AvrtBVHNode *nodes, uint32_t j, uint32_t *info)
+{
+    const AvrtBVHNode &n = nodes[j];
+    if (n.count > 0) {
+        if (n.count > static_cast<int32_t>(kMaxLeafCount) || static_cast<uint32_t>(n.left) + n.count > kMaxLeafFirst) {
+            atomicOr(&info[1], kErrLeaf);
+            return kRefInvalid;
+        }
+        return make_leaf(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count));
+    }
+    return 1u + j;
+}
+
+// Node2 index of reference node j is 1 + j; index 0 is the super-root.
+__global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, Node2 *out, uint32_t *info)
+{
+    const uint32_t nn = info_in[0];
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nn || nn == 0) return;
+    Node2 n;
+    const AvrtBVHNode far_pt{1e30f, 1e30f, 1e30f, 0.0f, 1e30f, 1e30f, 1e30f, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 2; c++) set_child(n, c, far_pt, kRefInvalid);
+    n.pad[0] = n.pad[1] = 0u;
+    if (j == 0) {
+        set_child(n, 0, nodes[0], ref_of(nodes, 0, info));
+    } else {
+        const AvrtBVHNode &me = nodes[j - 1];
+        if (me.count == 0) {
+            set_child(n, 0, nodes[me.left], ref_of(nodes, static_cast<uint32_t>(me.left), info));
+            set_child(n, 1, nodes[me.right], ref_of(nodes, static_cast<uint32_t>(me.right), info));
+        }
+    }
+    out[j] = n;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Carves the builder's work arrays out of one device buffer.
+BuildWork carve(void *base, uint32_t N, size_t *bytes_out)
+{
+    const size_t n = N ? N : 1, T = 2 * n;
+    size_t off = 0;
+    char *b = static_cast<char *>(base);
+    auto take = [&](size_t bytes) -> void * {
+        void *p = b ? b + off : nullptr;
+        off += align_up(bytes);
+        return p;
+    };
+    BuildWork w{};
+    w.idx = static_cast<uint32_t *>(take(4 * n));
+    w.idx2 = static_cast<uint32_t *>(take(4 * n));
+    w.flag = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.flag2 = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.scanA = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.scanB = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.posL = static_cast<uint32_t *>(take(4 * n));
+    w.posR = static_cast<uint32_t *>(take(4 * n));
+    w.fbeg = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.cen = static_cast<float *>(take(12 * n));
+    w.t_start = static_cast<uint32_t *>(take(4 * T));
+    w.t_end = static_cast<uint32_t *>(take(4 * T));
+    w.t_depth = static_cast<uint32_t *>(take(4 * T));
+    w.t_left = static_cast<uint32_t *>(take(4 * T));
+    w.t_right = static_cast<uint32_t *>(take(4 * T));
+    w.t_size = static_cast<uint32_t *>(take(4 * T));
+    w.t_pre = static_cast<uint32_t *>(take(4 * T));
+    w.t_leaf = static_cast<uint32_t *>(take(4 * T));
+    w.t_mid = static_cast<uint32_t *>(take(4 * T));
+    w.t_part = static_cast<uint32_t *>(take(4 * T));
+    w.t_kids = static_cast<uint32_t *>(take(4 * T));
+    w.t_axis = static_cast<int32_t *>(take(4 * T));
+    w.t_split = static_cast<int32_t *>(take(4 * T));
+    w.t_key = static_cast<unsigned long long *>(take(8 * 6 * T));
+    w.t_box = static_cast<float *>(take(4 * 6 * T));
+    w.t_cbox = static_cast<uint32_t *>(take(4 * 6 * T));
+    w.t_scale = static_cast<float *>(take(4 * 3 * T));
+    w.bins = static_cast<uint32_t *>(take(4 * n * 3 * 12 * 7));
+    *bytes_out = off;
+    return w;
+}
+
+inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
+{
+    SceneRebuild &rb = c->rb;
+    const uint32_t n = rb.n, m = rb.m;
+    HIPX(c, hipEventRecord(c->ev_a, c->stream));
+    if (vel_host && m) {
+        HIPX(c, hipMemcpyAsync(rb.vel.p, vel_host, sizeof(float) * 6 * m, hipMemcpyHostToDevice, c->stream));
+        if (dt != 0.0f) {
+            hipLaunchKernelGGL(k_integrate, dim3(blocks(m, 256)), dim3(256), 0, c->stream, m, dt, rb.vel.as<float>(),
+                               rb.auth_inst.as<AvrtMeshInstance>());
+            HIPX(c, hipGetLastError());
+            hipLaunchKernelGGL(k_sync_mesh_objects, dim3(blocks(n, 256)), dim3(256), 0, c->stream, n,
+                               rb.mesh_rank.as<int32_t>(), rb.auth_inst.as<AvrtMeshInstance>(),
+                               rb.auth_objects.as<AvrtObject>());
+            HIPX(c, hipGetLastError());
+        }
+    }
+    HIPX(c, hipMemsetAsync(rb.info.p, 0, 32, c->stream));
+    if (n) {
+        hipLaunchKernelGGL(k_bounds, dim3(blocks(n, 256)), dim3(256), 0, c->stream, n, rb.auth_objects.as<AvrtObject>(),
+                           rb.mesh_rank.as<int32_t>(), rb.auth_inst.as<AvrtMeshInstance>(), rb.local.as<AvrtAABB>(),
+                           rb.bounds.as<AvrtAABB>(), rb.info.as<uint32_t>());
+        HIPX(c, hipGetLastError());
+    }
+    size_t sbytes = 0;
+    BuildWork w = carve(rb.scratch.p, n, &sbytes);
+    hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kBT), 0, c->stream, n, rb.bounds.as<AvrtAABB>(), w,
+                       rb.nodes.as<AvrtBVHNode>(), rb.order.as<uint32_t>(), rb.info.as<uint32_t>());
+    HIPX(c, hipGetLastError());
+    if (n) {
+        hipLaunchKernelGGL(k_pack, dim3(1), dim3(kBT), 0, c->stream, n, rb.order.as<uint32_t>(),
+                           rb.auth_objects.as<AvrtObject>(), rb.mesh_rank.as<int32_t>(),
+                           rb.auth_inst.as<AvrtMeshInstance>(), rb.inst_geomc.as<uint4>(), c->d_objects.as<AvrtObject>(),
+                           c->d_instances.as<AvrtMeshInstance>(), c->d_geom.as<ObjGeom>(), c->d_lights.as<uint32_t>());
+        HIPX(c, hipGetLastError());
+        hipLaunchKernelGGL(k_convert_tlas, dim3(blocks(2 * n + 1, 256)), dim3(256), 0, c->stream, rb.info.as<uint32_t>(),
+                           rb.nodes.as<AvrtBVHNode>(), c->d_tlas2.as<Node2>(), rb.info.as<uint32_t>());
+        HIPX(c, hipGetLastError());
+    }
+    HIPX(c, hipEventRecord(c->ev_b, c->stream));
+    uint32_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPX(c, hipMemcpyAsync(info, rb.info.p, 32, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    (void)hipEventElapsedTime(&rb.last_ms, c->ev_a, c->ev_b);
+    if (info[1] & kErrBox) return set_err(c, AVRT_E_UNSUPPORTED, "device rebuild does not handle box objects");
+    if (info[1] & kErrLeaf) return set_err(c, AVRT_E_UNSUPPORTED, "device-built TLAS leaf too large for Node2");
+    if (info[1] & kErrBuild) return set_err(c, AVRT_E_DEVICE, "device TLAS build produced an inconsistent partition");
+    c->nobj = n;
+    c->ntlas2 = n ? info[0] + 1 : 0;
+    c->nlights = rb.nlights;
+    c->nquad_lights = rb.nquad_lights;
+    c->scene_loaded = true;
+    return AVRT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *objects_in, uint32_t object_count,
+                             const AvrtMeshInstance *instances_in, uint32_t instance_count, const AvrtAABB *local_bounds)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if ((object_count && !objects_in) || (instance_count && (!instances_in || !local_bounds)))
+        return set_err(c, AVRT_E_INVALID_ARG, "null array with non-zero count");
+    if (object_count >= kMaxLeafFirst) return set_err(c, AVRT_E_CAPACITY, "more than 2^24 objects");
+    std::vector<int32_t> rank(object_count, -1);
+    uint32_t meshes = 0, nl = 0, nq = 0;
+    bool tri_or_mesh = false;
+    for (uint32_t i = 0; i < object_count; i++) {
+        const AvrtObject &o = objects_in[i];
+        if (o.colorType == AVRT_ISOTROPIC) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
+        if (o.shapeType == AVRT_SHAPE_BOX) return set_err(c, AVRT_E_UNSUPPORTED, "device rebuild does not handle box objects");
+        if (o.shapeType > AVRT_SHAPE_MESH) return set_err(c, AVRT_E_INVALID_ARG, "unknown shape type");
+        if (o.shapeType == AVRT_SHAPE_MESH) rank[i] = static_cast<int32_t>(meshes++);
+        if (o.shapeType == AVRT_SHAPE_MESH || o.shapeType == AVRT_SHAPE_TRIANGLE) tri_or_mesh = true;
+        if (o.colorType == AVRT_DIFFUSE_LIGHT) {
+            nl++;
+            if (o.shapeType == AVRT_SHAPE_QUAD) nq++;
+        }
+    }
+    if (meshes != instance_count) return set_err(c, AVRT_E_INVALID_ARG, "one instance per mesh object is required");
+    HIPX(c, hipSetDevice(c->device));
+    std::vector<uint4> geomc(instance_count);
+    if (instance_count) {
+        if (!c->geometry_loaded) return set_err(c, AVRT_E_STATE, "mesh instances before avrt_upload_geometry");
+        bool changed = false;
+        int rc = prepare_meshes(c, instances_in, instance_count, &changed);
+        if (rc) return rc;
+        if (changed) {
+            rc = upload_mesh_tables(c);
+            if (rc) return rc;
+        }
+        for (uint32_t k = 0; k < instance_count; k++) {
+            const AvrtMeshInstance &m = instances_in[k];
+            const uint32_t root = m.triangleCount ? c->blas_root[m.blasBase] : 0u;
+            geomc[k] = make_uint4(root, m.indexBase / 3, m.vertexBase, m.indexBase);
+        }
+    }
+    SceneRebuild &rb = c->rb;
+    const size_t n = object_count, m = instance_count;
+    size_t sbytes = 0;
+    carve(nullptr, object_count, &sbytes);
+    int rc = 0;
+    if ((rc = upload_vec(c, rb.auth_objects, objects_in, sizeof(AvrtObject) * n))) return rc;
+    if ((rc = upload_vec(c, rb.auth_inst, instances_in, sizeof(AvrtMeshInstance) * m))) return rc;
+    if ((rc = upload_vec(c, rb.local, local_bounds, sizeof(AvrtAABB) * m))) return rc;
+    if ((rc = upload_vec(c, rb.inst_geomc, geomc.data(), sizeof(uint4) * m))) return rc;
+    if ((rc = upload_vec(c, rb.mesh_rank, rank.data(), sizeof(int32_t) * n))) return rc;
+    if ((rc = ensure(c, rb.vel, sizeof(float) * 6 * (m ? m : 1)))) return rc;
+    HIPX(c, hipMemsetAsync(rb.vel.p, 0, sizeof(float) * 6 * (m ? m : 1), c->stream));
+    if ((rc = ensure(c, rb.bounds, sizeof(AvrtAABB) * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, rb.nodes, sizeof(AvrtBVHNode) * 2 * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, rb.order, sizeof(uint32_t) * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, rb.scratch, sbytes))) return rc;
+    if ((rc = ensure(c, rb.info, 64))) return rc;
+    if ((rc = ensure(c, c->d_objects, sizeof(AvrtObject) * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, c->d_instances, sizeof(AvrtMeshInstance) * (m ? m : 1)))) return rc;
+    if ((rc = ensure(c, c->d_geom, sizeof(ObjGeom) * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, c->d_lights, sizeof(uint32_t) * (nl ? nl : 1)))) return rc;
+    if ((rc = ensure(c, c->d_tlas2, sizeof(Node2) * (2 * n + 1)))) return rc;
+    rb.n = object_count;
+    rb.m = instance_count;
+    rb.nlights = nl;
+    rb.nquad_lights = nq;
+    rb.ready = true;
+    c->has_volume = false;
+    c->has_tri_or_mesh = tri_or_mesh;
+    return run_rebuild(c, nullptr, 0.0f);
+}
+
+int avrt_scene_step(avrt_ctx *c, const float *velocities, float dt)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if (!c->rb.ready) return set_err(c, AVRT_E_STATE, "avrt_scene_step before avrt_scene_set_authoring");
+    HIPX(c, hipSetDevice(c->device));
+    return run_rebuild(c, velocities, dt);
+}
+
+int avrt_scene_read(avrt_ctx *c, AvrtBVHNode *nodes, uint32_t *node_count, AvrtObject *objects,
+                    AvrtMeshInstance *instances, uint32_t *order)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if (!c->rb.ready) return set_err(c, AVRT_E_STATE, "no device-built scene");
+    HIPX(c, hipSetDevice(c->device));
+    const uint32_t nn = c->ntlas2 ? c->ntlas2 - 1 : 0;
+    if (node_count) *node_count = nn;
+    if (nodes && nn) HIPX(c, hipMemcpyAsync(nodes, c->rb.nodes.p, sizeof(AvrtBVHNode) * nn, hipMemcpyDeviceToHost, c->stream));
+    if (objects && c->rb.n)
+        HIPX(c, hipMemcpyAsync(objects, c->d_objects.p, sizeof(AvrtObject) * c->rb.n, hipMemcpyDeviceToHost, c->stream));
+    if (instances && c->rb.m)
+        HIPX(c, hipMemcpyAsync(instances, c->d_instances.p, sizeof(AvrtMeshInstance) * c->rb.m, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (order && c->rb.n)
+        HIPX(c, hipMemcpyAsync(order, c->rb.order.p, sizeof(uint32_t) * c->rb.n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+int avrt_refit_blas(avrt_ctx *c) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *, uint32_t, uint32_t)
+{
+    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
+}
+int avrt_read_blas(avrt_ctx *c, AvrtBVHNode *, uint32_t) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+
+} // extern "C"

@@ -121,3 +121,30 @@ class Renderer:
         self._check(self.lib.avrt_trace_primary(self.ctx, c.ctypes.data_as(ctypes.c_void_p), ptr(obj), ptr(prim), ptr(t)),
                     "avrt_trace_primary")
         return obj, prim, t
+
+    # -- device-side scene rebuild (RebuildAcceleration on the GPU) ----------
+    def SetAuthoring(self, objects, instances, local_bounds):
+        """Objects in authoring (GatherRenderables) order, one instance record
+        per mesh object in the same order, and each instance's mesh-local
+        AABB. Builds the TLAS on the device."""
+        o = np.ascontiguousarray(objects, abi.OBJECT_DTYPE)
+        m = np.ascontiguousarray(instances, abi.INSTANCE_DTYPE)
+        lb = np.ascontiguousarray(local_bounds, abi.AABB_DTYPE)
+        self._check(self.lib.avrt_scene_set_authoring(self.ctx, ptr(o), len(o), ptr(m), len(m), ptr(lb)),
+                    "avrt_scene_set_authoring")
+        self._n_auth, self._m_auth = len(o), len(m)
+
+    def SceneStep(self, velocities=None, dt=0.0):
+        v = None if velocities is None else np.ascontiguousarray(velocities, np.float32)
+        self._check(self.lib.avrt_scene_step(self.ctx, ptr(v), ctypes.c_float(dt)), "avrt_scene_step")
+
+    def SceneRead(self):
+        n, m = self._n_auth, self._m_auth
+        nodes = np.zeros(max(2 * n, 1), abi.NODE_DTYPE)
+        cnt = ctypes.c_uint32(0)
+        objs = np.zeros(max(n, 1), abi.OBJECT_DTYPE)
+        inst = np.zeros(max(m, 1), abi.INSTANCE_DTYPE)
+        order = np.zeros(max(n, 1), np.uint32)
+        self._check(self.lib.avrt_scene_read(self.ctx, ptr(nodes), ctypes.byref(cnt), ptr(objs), ptr(inst), ptr(order)),
+                    "avrt_scene_read")
+        return nodes[:cnt.value].copy(), objs[:n].copy(), inst[:m].copy(), order[:n].copy()

@@ -128,19 +128,24 @@ def perlin_texture(size: int, seed: int, octaves: int = 4, cells: int = 8) -> np
     amp, norm = 1.0, 0.0
     for o in range(octaves):
         period = cells * (2 ** o)
+        # one gradient per lattice point of this octave (the texture tiles)
+        gy_, gx_ = np.meshgrid(np.arange(period), np.arange(period), indexing="ij")
+        h = _hash_u32(gx_, gy_, np.int64(seed + 31 * o))
+        ang = (h & np.uint32(0xFFFF)).astype(np.float32) * np.float32(2 * math.pi / 65536.0)
+        gcos, gsin = np.cos(ang), np.sin(ang)
         px, py = x * period, y * period
         ix, iy = np.floor(px).astype(np.int64), np.floor(py).astype(np.int64)
         fx, fy = (px - ix).astype(np.float32), (py - iy).astype(np.float32)
+        ix0, iy0 = ix % period, iy % period
+        ix1, iy1 = (ix + 1) % period, (iy + 1) % period
 
         def grad(gx, gy, dx, dy):
-            h = _hash_u32(gx % period, gy % period, np.int64(seed + 31 * o))
-            ang = (h & np.uint32(0xFFFF)).astype(np.float32) * np.float32(2 * math.pi / 65536.0)
-            return np.cos(ang) * dx + np.sin(ang) * dy
+            return gcos[gy, gx] * dx + gsin[gy, gx] * dy
 
-        n00 = grad(ix, iy, fx, fy)
-        n10 = grad(ix + 1, iy, fx - 1, fy)
-        n01 = grad(ix, iy + 1, fx, fy - 1)
-        n11 = grad(ix + 1, iy + 1, fx - 1, fy - 1)
+        n00 = grad(ix0, iy0, fx, fy)
+        n10 = grad(ix1, iy0, fx - 1, fy)
+        n01 = grad(ix0, iy1, fx, fy - 1)
+        n11 = grad(ix1, iy1, fx - 1, fy - 1)
         u = fx * fx * fx * (fx * (fx * 6 - 15) + 10)
         v = fy * fy * fy * (fy * (fy * 6 - 15) + 10)
         nx0 = n00 + u * (n10 - n00)
@@ -148,15 +153,16 @@ def perlin_texture(size: int, seed: int, octaves: int = 4, cells: int = 8) -> np
         acc += amp * (nx0 + v * (nx1 - nx0))
         norm += amp
         amp *= 0.5
-        del n00, n10, n01, n11, px, py, ix, iy, fx, fy, u, v, nx0, nx1
-    n = np.clip(acc / norm * 1.4 + 0.5, 0.0, 1.0)
-    rgb = np.stack([0.25 + 0.7 * n, 0.2 + 0.6 * n * n, 0.55 - 0.35 * n], axis=-1)
-    rgb = np.clip(rgb, 0.0, 1.0)
+        del n00, n10, n01, n11, px, py, ix, iy, fx, fy, u, v, nx0, nx1, ix0, iy0, ix1, iy1
+    n = np.clip(acc * np.float32(1.4 / norm) + np.float32(0.5), 0.0, 1.0)
+    # noise value -> sRGB-encoded colour through a 4096-entry palette
+    t = np.linspace(0.0, 1.0, 4096)
+    rgb = np.clip(np.stack([0.25 + 0.7 * t, 0.2 + 0.6 * t * t, 0.55 - 0.35 * t], axis=-1), 0.0, 1.0)
     enc = np.where(rgb <= 0.0031308, rgb * 12.92, 1.055 * np.power(rgb, 1 / 2.4) - 0.055)
-    out = np.empty((size, size, 4), np.uint8)
-    out[..., :3] = np.round(enc * 255.0).astype(np.uint8)
-    out[..., 3] = 255
-    return out
+    palette = np.empty((4096, 4), np.uint8)
+    palette[:, :3] = np.round(enc * 255.0).astype(np.uint8)
+    palette[:, 3] = 255
+    return palette[(n * np.float32(4095.0) + np.float32(0.5)).astype(np.int32)]
 
 
 # ---------------------------------------------------------------------------
@@ -573,16 +579,18 @@ def object_bounds(objects: np.ndarray, instances: np.ndarray | None, inst_local:
     s = shape == SPHERE
     lo[s] = P[s] - R[s, None]
     hi[s] = P[s] + R[s, None]
+    smin = lambda a, b: np.where(b < a, b, a)  # std::min: equal values keep the first
+    smax = lambda a, b: np.where(a < b, b, a)  # std::max
     q = shape == QUAD
     if q.any():
         c0, c1, c2, c3 = P[q], (P[q] + U[q]) + V[q], P[q] + U[q], P[q] + V[q]
-        lo[q] = np.minimum(np.minimum(c0, c1), np.minimum(c2, c3))
-        hi[q] = np.maximum(np.maximum(c0, c1), np.maximum(c2, c3))
+        lo[q] = smin(smin(c0, c1), smin(c2, c3))
+        hi[q] = smax(smax(c0, c1), smax(c2, c3))
     t = shape == TRIANGLE
     if t.any():
         c0, c1, c2 = P[t], P[t] + U[t], P[t] + V[t]
-        lo[t] = np.minimum(np.minimum(c0, c1), c2)
-        hi[t] = np.maximum(np.maximum(c0, c1), c2)
+        lo[t] = smin(smin(c0, c1), c2)
+        hi[t] = smax(smax(c0, c1), c2)
     m = np.nonzero(shape == MESH)[0]
     if len(m):
         wb = builder.instance_bounds(instances, inst_local)
@@ -603,6 +611,15 @@ def full_instances(spec: SceneSpec, ranges) -> np.ndarray:
         vb, ib, tc, bb = ranges[m]
         inst[k]["vertexBase"], inst[k]["indexBase"], inst[k]["triangleCount"], inst[k]["blasBase"] = vb, ib, tc, bb
     return inst
+
+
+def authoring(spec: SceneSpec, library) -> tuple:
+    """(objects, instances, local bounds) in authoring order, the input of
+    the device-side rebuild (avrt_scene_set_authoring)."""
+    verts, idx, blas, ranges, local = library
+    inst = full_instances(spec, ranges)
+    loc = np.array([local[m] for m in spec.instance_mesh], AABB_DTYPE) if len(inst) else np.zeros(0, AABB_DTYPE)
+    return spec.objects.copy(), inst, loc
 
 
 def pack(spec: SceneSpec, builder: Builder | None = None, library=None) -> Packed:

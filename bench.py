@@ -5,10 +5,13 @@ Workload (default): the two-level instanced scene at 1920x1080, 8 samples
 per pixel, depth 8 - 1024 instances of four seeded prototype meshes (5,120 /
 12,288 / 20,000 / 131,072 triangles), one 4096^2 RGBA8 Perlin albedo
 texture, one emissive sphere - rendered with the ray_trace (hybrid
-compute-only) integrator. A step is one frame; its rays (camera, bounce and
-shadow queries) over the frame time give Mrays/s. Scene data is resident in
-HBM before the timed region; each frame runs raygen -> [extend -> shade ->
-shadow] x depth -> resolve on the device.
+compute-only) integrator. A step is one frame of the animated scene: the
+instances move as rigid bodies (linear and angular velocity, dt = 1/60), the
+TLAS is rebuilt on the device, then the frame is traced. Its rays (camera,
+bounce and shadow queries) over the wall time of the step give Mrays/s. Scene
+data is resident in HBM before the timed region; each frame runs integrate ->
+bounds -> TLAS build -> pack, then raygen -> [extend -> shade -> shadow] x
+depth -> resolve, all on the device.
 
 Prints one JSON line (see the end of main) with the roofline of the dominant
 kernel (closest-hit traversal) and the CPU oracle timed on this host.
@@ -72,14 +75,21 @@ def main():
     t0 = time.perf_counter()
     spec = scenes.instanced(width=args.width, height=args.height, spp=args.spp, depth=args.depth,
                             n_instances=args.instances, tex_size=args.tex)
-    packed = scenes.pack(spec)
+    builder = scenes.Builder()
+    library = scenes.mesh_library(spec.meshes, builder)
+    packed = scenes.pack(spec, builder, library)
+    objs, inst, local = scenes.authoring(spec, library)
     cfg = packed.config.copy()
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {len(packed.objects)} objects, "
         f"{len(packed.indices) // 3} BLAS triangles, {len(packed.blas)} BLAS nodes")
 
     r = Renderer().Initialize(device=0)
-    r.LoadScene(packed)
+    r.UploadMeshGeometry(library[0], library[1], library[2])
+    r.UploadTextures(spec.texture)
+    r.SetAuthoring(objs, inst, local)
+    dt = 1.0 / 60.0
     for w in range(args.warmup):
+        r.SceneStep(spec.velocities, dt)
         s = r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
         log(f"warmup {w}: {s['ms_total']:.1f} ms")
 
@@ -87,6 +97,7 @@ def main():
            "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "launches_extend": 0, "ms_total": 0.0}
     t_start = time.perf_counter()
     for k in range(args.steps):
+        r.SceneStep(spec.velocities, dt)
         s = r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
         for key in tot:
             tot[key] += s[key]
@@ -135,8 +146,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "two-level instanced scene, plain form", "width": args.width, "height": args.height,
-                   "spp": args.spp, "depth": args.depth, "instances": args.instances, "texture": args.tex},
+        "config": {"workload": "two-level instanced scene, rigid motion + device TLAS rebuild per frame, plain form",
+                   "width": args.width, "height": args.height, "spp": args.spp, "depth": args.depth,
+                   "instances": args.instances, "texture": args.tex},
         "samples_per_s": samples_per_s,
         "rays": {"primary": tot["rays_primary"], "bounce": tot["rays_bounce"], "shadow": tot["rays_shadow"]},
         "device_ms_per_step": {k: tot[k] / args.steps for k in ("ms_total", "ms_extend", "ms_shade", "ms_shadow",

@@ -208,7 +208,9 @@ typedef struct AvrtStats {
     float ms_shadow;         /* device time in shadow traversal        */
     float ms_other;          /* raygen, resolve, TLAS rebuild          */
     uint32_t launches_extend;/* number of extend launches              */
-    uint32_t reserved[3];
+    float ms_scene_step;     /* device time of the last avrt_scene_step
+                                or avrt_scene_set_authoring rebuild      */
+    uint32_t reserved[2];
 } AvrtStats;
 
 typedef struct avrt_ctx avrt_ctx;
