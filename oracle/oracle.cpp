@@ -874,6 +874,331 @@ static inline float round_half(float x)
     return sign ? -r : r;
 }
 
+
+// ---------------------------------------------------------------------------
+// Hybrid frame: G-buffer, deferred lighting, temporal upsample
+// (shaders/gbuffer.{vert,frag}.hlsl, deferred.comp.hlsl, upsample.comp.hlsl).
+// ---------------------------------------------------------------------------
+
+// exp(x): x = k ln2 + r (ln2 split in two, Cody-Waite), Taylor degree 7 on
+// |r| <= ln2/2, scaled by 2^k. Shared definition with the HIP path.
+static inline float rt_exp(float x)
+{
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return INFINITY;
+    const float k = std::nearbyint(x * 1.44269504f);
+    float r = x - k * 0.693145752f;
+    r = r - k * 1.42860677e-6f;
+    float p = 1.98412698e-4f;
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return std::ldexp(p, static_cast<int>(k));
+}
+
+static inline float pow32(float x)
+{
+    float y = x * x;
+    y = y * y;
+    y = y * y;
+    y = y * y;
+    return y * y;
+}
+
+// IsRasterProxy (renderer.h:317, deferred.comp.hlsl:1011-1017).
+static inline bool is_raster_proxy(const Object &o)
+{
+    return o.shapeType == BOX && o.colorType != ISOTROPIC && o.textureID == NO_TEXTURE && o.rotation == 0.0f;
+}
+static inline bool is_rasterized(const Object &o) { return o.shapeType == MESH || is_raster_proxy(o); }
+
+// Closest hit within [near, far_limit] visiting the nearer child first
+// (TraverseBVHBounded, deferred.comp.hlsl:1027-1099). filter 1 keeps only what
+// the rasterizer draws (the G-buffer); filter 2 skips it (ResolvePrimary).
+static bool traverse_bounded(const Scene &s, const Ray &r, float near, float far_limit, int filter, Hit &hit)
+{
+    if (s.nnode == 0) return false;
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    bool found = false;
+    float far = far_limit;
+    while (sp > 0) {
+        const Node &n = s.bvh[stack[--sp]];
+        if (!intersect_aabb(r.o, inv, n, near, far)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; i++) {
+                const uint32_t id = static_cast<uint32_t>(n.left + i);
+                const bool ras = is_rasterized(s.objects[id]);
+                if ((filter == 1 && !ras) || (filter == 2 && ras)) continue;
+                Hit h;
+                if (hit_object(s, id, r, near, far, h, nullptr)) {
+                    far = h.t;
+                    hit = h;
+                    found = true;
+                }
+            }
+        } else {
+            const Node &L = s.bvh[n.left], &R = s.bvh[n.right];
+            auto entry = [&](const Node &b) {
+                const V3 t0 = (v3(b.min_x, b.min_y, b.min_z) - r.o) * inv;
+                const V3 t1 = (v3(b.max_x, b.max_y, b.max_z) - r.o) * inv;
+                const V3 ts = vmin(t0, t1), tb = vmax(t0, t1);
+                const float tmin = fmaxn(fmaxn(ts.x, ts.y), fmaxn(ts.z, near));
+                const float tmax = fminn(fminn(tb.x, tb.y), fminn(tb.z, far));
+                return (tmin <= tmax) ? tmin : 1e30f;
+            };
+            const float tl = entry(L), tr = entry(R);
+            const bool left_first = tl <= tr;
+            const int nearc = left_first ? n.left : n.right;
+            const int farc = left_first ? n.right : n.left;
+            if (fmaxn(tl, tr) < 1e30f) stack[sp++] = farc;
+            if (fminn(tl, tr) < 1e30f) stack[sp++] = nearc;
+        }
+    }
+    return found;
+}
+
+// Any-hit: is anything but glass or a light in [near, far]
+// (Occluded / OccludedMesh, deferred.comp.hlsl:1115-1171, 812-854).
+static bool occluded(const Scene &s, const Ray &r, float near, float far, Counters *cnt)
+{
+    if (cnt) cnt->rays++;
+    if (s.nnode == 0) return false;
+    const V3 inv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const Node &n = s.bvh[stack[--sp]];
+        if (cnt) cnt->nodes++;
+        if (!intersect_aabb(r.o, inv, n, near, far)) continue;
+        if (n.count > 0) {
+            for (int i = 0; i < n.count; i++) {
+                const uint32_t id = static_cast<uint32_t>(n.left + i);
+                const Object &o = s.objects[id];
+                if (o.colorType == DIELECTRIC || o.colorType == DIFFUSE_LIGHT) continue;
+                if (cnt) cnt->objs++;
+                if (o.shapeType == MESH) {
+                    const Inst &in = s.inst[o.instanceIndex];
+                    const float scale = (in.scale == 0.0f) ? 1.0f : in.scale;
+                    Ray lr;
+                    lr.o = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.o - v3(in.px, in.py, in.pz));
+                    lr.d = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.d);
+                    if (scale != 1.0f) {
+                        lr.o = lr.o / scale;
+                        lr.d = lr.d / scale;
+                    }
+                    const V3 li = v3(1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z);
+                    int st2[64];
+                    int sp2 = 0;
+                    st2[sp2++] = 0;
+                    while (sp2 > 0) {
+                        const Node &b = s.blas[in.blasBase + st2[--sp2]];
+                        if (cnt) cnt->nodes++;
+                        if (!intersect_aabb(lr.o, li, b, near, far)) continue;
+                        if (b.count > 0) {
+                            for (int k = 0; k < b.count; k++) {
+                                const uint32_t b3 = in.indexBase + static_cast<uint32_t>(b.left + k) * 3u;
+                                float t, al, be;
+                                V3 g;
+                                if (cnt) cnt->tris++;
+                                if (hit_mesh_tri(lr, near, far, s.verts[in.vertexBase + s.idx[b3]],
+                                                 s.verts[in.vertexBase + s.idx[b3 + 1]],
+                                                 s.verts[in.vertexBase + s.idx[b3 + 2]], &t, &al, &be, &g))
+                                    return true;
+                            }
+                        } else {
+                            st2[sp2++] = b.left;
+                            st2[sp2++] = b.right;
+                        }
+                    }
+                    continue;
+                }
+                Hit h;
+                if (hit_object(s, id, r, near, far, h, nullptr)) return true;
+            }
+        } else {
+            stack[sp++] = n.left;
+            stack[sp++] = n.right;
+        }
+    }
+    return false;
+}
+
+// CalculateDirectLight of deferred.comp (:1376-1437): one bounded any-hit
+// shadow query per light, clamped like ClampedDirectLight (:1556-1569).
+static V3 clamped_direct_deferred(const Scene &s, const Hit &hit, V3 brdf, const Ray &ray, Rng &rng, Counters *cnt)
+{
+    V3 sum = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < s.nlights; i++) {
+        const Object &l = s.objects[s.lights[i]];
+        V3 lp;
+        float solid;
+        sample_light(l, hit.pos, rng, &lp, &solid);
+        if (solid <= 0.0f) continue;
+        V3 dir = lp - hit.pos;
+        const float dist = length(dir);
+        if (dist <= 1e-6f) continue;
+        dir = dir / dist;
+        const float ct = dot(hit.normal, dir);
+        if (ct <= 0.0f) continue;
+        Ray sr;
+        sr.o = hit.pos + (hit.normal * 0.001f);
+        sr.d = dir;
+        sr.time = ray.time;
+        if (!occluded(s, sr, 0.001f, dist - 0.001f, cnt ? cnt + 1 : nullptr)) {
+            const V3 radiance = l.emission * albedo_of(l);
+            sum = sum + brdf * radiance * ct * solid;
+        }
+    }
+    const float luma = fmaxn(sum.x, fmaxn(sum.y, sum.z));
+    if (luma > 8.0f) sum = sum * (8.0f / luma);
+    return sum;
+}
+
+// An object with its albedo replaced by the G-buffer's and its texture cleared
+// (HitFromGBuffer, deferred.comp.hlsl:1452-1493).
+struct Surface {
+    Hit hit;
+    uint32_t color_type;
+    V3 albedo;      // SurfaceAlbedo of the hit
+    bool from_gbuffer;
+};
+
+static V3 trace_indirect(const Scene &s, const Ray &primary, const Surface &sf, uint32_t depth, V3 sky, Rng &rng,
+                         Counters *cnt)
+{
+    const Object &o = s.objects[sf.hit.obj];
+    Object oo = o;
+    oo.colorType = sf.color_type;
+    V3 bd;
+    if (!get_bounce(oo, primary, sf.hit, rng, &bd)) return v3(0.0f, 0.0f, 0.0f);
+    Ray ray;
+    ray.d = normalize(bd);
+    const float sign = (dot(ray.d, sf.hit.normal) < 0.0f) ? -1.0f : 1.0f;
+    ray.o = sf.hit.pos + sf.hit.normal * (0.001f * sign);
+    ray.time = primary.time;
+    V3 tp = (sf.color_type != DIELECTRIC) ? sf.albedo : v3(1.0f, 1.0f, 1.0f);
+    V3 acc = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t b = 1; b < depth; b++) {
+        Hit hit;
+        if (cnt) cnt->rays++;
+        if (!traverse_bounded(s, ray, 0.001f, 1000000.0f, 0, hit)) {
+            acc = acc + tp * sky;
+            break;
+        }
+        const Object &h = s.objects[hit.obj];
+        if (h.colorType == DIFFUSE_LIGHT) {
+            acc = acc + tp * (h.emission * albedo_of(h));
+            break;
+        }
+        const V3 salb = surface_albedo_hybrid(s, hit);
+        if (h.colorType == LAMBERTIAN) acc = acc + tp * clamped_direct_deferred(s, hit, salb, ray, rng, cnt);
+        V3 nx;
+        if (!get_bounce(h, ray, hit, rng, &nx)) break;
+        ray.d = normalize(nx);
+        const float ls = (dot(ray.d, hit.normal) < 0.0f) ? -1.0f : 1.0f;
+        ray.o = hit.pos + hit.normal * (0.001f * ls);
+        if (h.colorType != DIELECTRIC) tp = tp * salb;
+        float p = fmaxn(tp.x, fmaxn(tp.y, tp.z));
+        p = fminn(fmaxn(p, 0.1f), 1.0f);
+        if (rng.next() > p) break;
+        tp = tp / p;
+    }
+    return acc;
+}
+
+struct HybridParams {
+    float jitter_x, jitter_y;
+    float trace_scale;
+    uint32_t frame_index;
+    uint32_t history_reset, camera_moved;
+    float history_cap;
+    float prev_view_proj[16];
+    float tan_half_fov;
+};
+
+static inline void traced_texel(uint32_t cx, uint32_t cy, uint32_t frame, float scale, uint32_t W, uint32_t H,
+                                uint32_t *tx, uint32_t *ty)
+{
+    static const float off[4][2] = {{0.0f, 0.0f}, {0.5f, 0.5f}, {0.5f, 0.0f}, {0.0f, 0.5f}};
+    const float fx = std::floor((static_cast<float>(cx) + off[frame % 4][0]) / scale);
+    const float fy = std::floor((static_cast<float>(cy) + off[frame % 4][1]) / scale);
+    *tx = std::min(static_cast<uint32_t>(fx), W - 1);
+    *ty = std::min(static_cast<uint32_t>(fy), H - 1);
+}
+
+// ReprojectHistory (deferred.comp.hlsl:1810-1875 = upsample.comp.hlsl:218-278).
+static void reproject(const Config &cfg, const HybridParams &hp, uint32_t px, uint32_t py, V3 color, bool surface,
+                      V3 position, V3 prev_position, V3 normal, bool view_dep, const float *prev_image,
+                      const float *prev_geom, V3 *result, float *count)
+{
+    const uint32_t W = cfg.width, H = cfg.height;
+    int hx = static_cast<int>(px), hy = static_cast<int>(py);
+    bool valid = hp.history_reset == 0;
+    const V3 src = v3(cfg.source[0], cfg.source[1], cfg.source[2]);
+    if (valid && hp.camera_moved != 0) {
+        valid = false;
+        if (surface) {
+            const float *m = hp.prev_view_proj;
+            const float cx = m[0] * prev_position.x + m[1] * prev_position.y + m[2] * prev_position.z + m[3] * 1.0f;
+            const float cy = m[4] * prev_position.x + m[5] * prev_position.y + m[6] * prev_position.z + m[7] * 1.0f;
+            const float cw = m[12] * prev_position.x + m[13] * prev_position.y + m[14] * prev_position.z + m[15] * 1.0f;
+            if (cw > 0.0f) {
+                const float u = (cx / cw) * 0.5f + 0.5f;
+                const float v = (cy / cw) * -0.5f + 0.5f;
+                hx = static_cast<int>(std::floor(u * static_cast<float>(W)));
+                hy = static_cast<int>(std::floor(v * static_cast<float>(H)));
+                if (hx >= 0 && hy >= 0 && hx < static_cast<int>(W) && hy < static_cast<int>(H)) {
+                    const float *pg = prev_geom + 4 * (static_cast<size_t>(hy) * W + hx);
+                    const V3 prev = v3(pg[0], pg[1], pg[2]);
+                    const float plane = std::fabs(dot(prev - prev_position, normal));
+                    const float tol = 0.01f * length(position - src) + 0.01f;
+                    const float footprint = length(position - src) * 2.0f * hp.tan_half_fov / static_cast<float>(H);
+                    const float facing = fmaxn(std::fabs(dot(normal, normalize(src - position))), 0.33f);
+                    const float drift = length(prev - prev_position);
+                    valid = pg[3] > 0.0f && plane < tol && drift < fmaxn(2.5f * footprint / facing, 0.01f);
+                }
+            }
+        }
+    }
+    if (!valid) {
+        *count = 1.0f;
+        *result = color;
+        return;
+    }
+    const size_t hidx = static_cast<size_t>(hy) * W + hx;
+    const float pc = std::fabs(prev_geom[4 * hidx + 3]);
+    const float moving = view_dep ? fminn(hp.history_cap, 4.0f) : hp.history_cap;
+    const float cap = (hp.camera_moved != 0) ? moving : 1e7f;
+    *count = fminn(pc + 1.0f, cap);
+    const V3 prev = v3(prev_image[4 * hidx], prev_image[4 * hidx + 1], prev_image[4 * hidx + 2]);
+    *result = lerp(prev, color, 1.0f / *count);
+}
+
+// CurrentRotation / PreviousRotation / PreviousPosition (deferred.comp.hlsl:1759-1793).
+static V3 previous_position(const Scene &s, const Object &o, V3 p)
+{
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f, rw = 1.0f;
+    if (o.shapeType == MESH) {
+        const Inst &in = s.inst[o.instanceIndex];
+        rx = in.qx; ry = in.qy; rz = in.qz; rw = in.qw;
+    } else if (o.shapeType == SPHERE || o.shapeType == BOX) {
+        const float h = 0.5f * o.rotation;
+        ry = std::sin(h);
+        rw = std::cos(h);
+    }
+    const V3 local = quat_rotate(-rx, -ry, -rz, rw, p - pos_of(o));
+    const V3 pv = v3(o.prevRotX, o.prevRotY, o.prevRotZ);
+    const float pw = std::sqrt(fmaxn(0.0f, 1.0f - dot(pv, pv)));
+    return quat_rotate(pv.x, pv.y, pv.z, pw, local) + pos2_of(o);
+}
 } // namespace orc
 
 using namespace orc;
