@@ -81,6 +81,11 @@ struct avrt_ctx {
     // path buffers
     avrt::DevBuf so[2], sd[2], st[2], hitA, hitB, Lpath, shp, shn, sha, shw, counters;
     size_t cap_paths = 0;
+    // shadow-ray wavefront (NEE) buffers
+    avrt::DevBuf sro[2], srd[2], srl[2], nee_tp, nee_contrib;
+    size_t cap_shadow = 0;
+    int cu_count = 0;
+    int trace_blocks_per_cu = 0;
 
     avrt::SceneRebuild rb;
 

@@ -299,42 +299,78 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
     return true;
 }
 
-// Closest hit through TLAS and BLAS. REF_ORDER=true visits children like the
-// reference (right child first), so equal-t ties resolve exactly as there;
-// false visits the nearer child first.
+// Traversal of one ray as a resumable state machine, so a persistent kernel
+// can refill a lane with a new ray as soon as its current one finishes.
+// step() performs one iteration (an interior node, a leaf, or the return
+// from a BLAS) and returns false once the ray is finished; the closest hit is
+// then in `hit`. REF_ORDER=true visits children like the reference (right
+// child first), so hits at exactly equal t resolve as there; false visits the
+// nearer child first.
 template <bool REF_ORDER, int K, int BLOCK, int SPILL>
-__device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, float time, float near, float far,
-                                              uint32_t *lds_stack, HitRec &hit, uint32_t *node_visits)
-{
-    hit.obj = ~0u;
-    hit.prim = ~0u;
-    hit.t = far;
-    if (sc.tlas == nullptr) return false;
-    const F3 winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    F3 ao = o, ad = d, ainv = winv;
+struct Traversal {
+    F3 o, d, winv;    // world ray
+    F3 ao, ainv;      // active ray (world, or mesh-local inside a BLAS)
+    F3 ad;
+    float time, near, far;
+    uint32_t ref;
+    uint32_t cur_obj, tri_base;
+    bool in_blas;
+    uint32_t steps;
+    HitRec hit;
     ShortStack<K, BLOCK, SPILL> st;
-    st.init(lds_stack);
-    bool in_blas = false;
-    uint32_t cur_obj = 0, tri_base = 0;
-    bool found = false;
-    uint32_t ref = 0; // TLAS super-root
-    uint32_t visits = 0;
-    // Every node of an acyclic tree is entered at most once per instance
-    // entry; the cap only bounds the loop should a malformed tree reach it.
-    uint32_t steps = 0;
-    while (++steps < kMaxTraversalSteps) {
+
+    __device__ __forceinline__ bool init(const SceneDev &sc, F3 o_, F3 d_, float time_, float near_, float far_,
+                                         uint32_t *lds_stack)
+    {
+        o = o_;
+        d = d_;
+        time = time_;
+        near = near_;
+        far = far_;
+        hit.obj = ~0u;
+        hit.prim = ~0u;
+        hit.t = far_;
+        hit.alpha = 0.0f;
+        hit.beta = 0.0f;
+        winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        ao = o;
+        ad = d;
+        ainv = winv;
+        st.init(lds_stack);
+        in_blas = false;
+        cur_obj = 0;
+        tri_base = 0;
+        ref = 0; // TLAS super-root
+        steps = 0;
+        return sc.tlas != nullptr;
+    }
+
+    // Pops the next entry; false when the stack is empty (ray finished).
+    __device__ __forceinline__ bool next()
+    {
+        if (st.empty()) return false;
+        ref = st.pop();
+        return true;
+    }
+
+    __device__ __forceinline__ bool step(const SceneDev &sc)
+    {
+        // Every node of an acyclic tree is entered at most once per instance
+        // entry; the cap only bounds the loop should a malformed tree reach it.
+        if (++steps >= kMaxTraversalSteps) return false;
         if (ref == kRefReturn) {
             in_blas = false;
             ao = o;
             ad = d;
             ainv = winv;
-        } else if ((ref & kLeafFlag) == 0) {
+            return next();
+        }
+        if ((ref & kLeafFlag) == 0) {
             const Node2 *np = (in_blas ? sc.blas : sc.tlas) + ref;
             const float4 X = *reinterpret_cast<const float4 *>(np->bx);
             const float4 Y = *reinterpret_cast<const float4 *>(np->by);
             const float4 Z = *reinterpret_cast<const float4 *>(np->bz);
             const uint4 R = *reinterpret_cast<const uint4 *>(np->ref);
-            visits++;
             float e0, e1;
             const bool h0 = slab(X.x, X.z, Y.x, Y.z, Z.x, Z.z, ao, ainv, near, far, &e0) && R.x != kRefInvalid;
             const bool h1 = slab(X.y, X.w, Y.y, Y.w, Z.y, Z.w, ao, ainv, near, far, &e1) && R.y != kRefInvalid;
@@ -342,108 +378,115 @@ __device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, fl
                 const bool first1 = REF_ORDER ? true : (e1 < e0);
                 st.push(first1 ? R.x : R.y);
                 ref = first1 ? R.y : R.x;
-                continue;
+                return true;
             }
             if (h0) {
                 ref = R.x;
-                continue;
+                return true;
             }
             if (h1) {
                 ref = R.y;
-                continue;
+                return true;
             }
-        } else {
-            const uint32_t first = ref & 0x00FFFFFFu;
-            const uint32_t count = (ref >> 24) & 0x7Fu;
-            if (in_blas) {
-                for (uint32_t i = 0; i < count; i++) {
-                    const TriPre *tp = sc.tris + tri_base + first + i;
-                    const float4 A = *reinterpret_cast<const float4 *>(tp->p0);
-                    const float4 B = *reinterpret_cast<const float4 *>(tp->e1);
-                    const float4 C = *reinterpret_cast<const float4 *>(tp->e2);
-                    const float4 W = *reinterpret_cast<const float4 *>(tp->w);
-                    if (__float_as_uint(B.w) == 0u) continue;
-                    const F3 e1 = f3(B.x, B.y, B.z), e2 = f3(C.x, C.y, C.z);
-                    const F3 n = cross(e1, e2);
-                    const float denom = dot(n, ad);
-                    if (fabsf(denom) < 1e-12f) continue;
-                    const float t = (A.w - dot(n, ao)) / denom;
-                    if (t < near || t > far) continue;
-                    const F3 planar = ao + ad * t - f3(A.x, A.y, A.z);
-                    const F3 w = f3(W.x, W.y, W.z);
-                    const float al = dot(w, cross(planar, e2));
-                    const float be = dot(w, cross(e1, planar));
-                    if (al < 0.0f || be < 0.0f || al + be > 1.0f) continue;
-                    far = t;
-                    found = true;
-                    hit.t = t;
-                    hit.obj = cur_obj;
-                    hit.prim = first + i;
-                    hit.alpha = al;
-                    hit.beta = be;
+            return next();
+        }
+        const uint32_t first = ref & 0x00FFFFFFu;
+        const uint32_t count = (ref >> 24) & 0x7Fu;
+        if (in_blas) {
+            for (uint32_t i = 0; i < count; i++) {
+                const TriPre *tp = sc.tris + tri_base + first + i;
+                const float4 A = *reinterpret_cast<const float4 *>(tp->p0);
+                const float4 B = *reinterpret_cast<const float4 *>(tp->e1);
+                const float4 C = *reinterpret_cast<const float4 *>(tp->e2);
+                const float4 W = *reinterpret_cast<const float4 *>(tp->w);
+                if (__float_as_uint(B.w) == 0u) continue;
+                const F3 e1 = f3(B.x, B.y, B.z), e2 = f3(C.x, C.y, C.z);
+                const F3 n = cross(e1, e2);
+                const float denom = dot(n, ad);
+                if (fabsf(denom) < 1e-12f) continue;
+                const float t = (A.w - dot(n, ao)) / denom;
+                if (t < near || t > far) continue;
+                const F3 planar = ao + ad * t - f3(A.x, A.y, A.z);
+                const F3 w = f3(W.x, W.y, W.z);
+                const float al = dot(w, cross(planar, e2));
+                const float be = dot(w, cross(e1, planar));
+                if (al < 0.0f || be < 0.0f || al + be > 1.0f) continue;
+                far = t;
+                hit.t = t;
+                hit.obj = cur_obj;
+                hit.prim = first + i;
+                hit.alpha = al;
+                hit.beta = be;
+            }
+            return next();
+        }
+        for (uint32_t i = 0; i < count; i++) {
+            const uint32_t slot = first + i;
+            const ObjGeom *gp = sc.geom + slot;
+            ObjGeom g;
+            *reinterpret_cast<float4 *>(g.a) = *reinterpret_cast<const float4 *>(gp->a);
+            *reinterpret_cast<float4 *>(g.b) = *reinterpret_cast<const float4 *>(gp->b);
+            *reinterpret_cast<float4 *>(g.c) = *reinterpret_cast<const float4 *>(gp->c);
+            *reinterpret_cast<uint4 *>(g.d) = *reinterpret_cast<const uint4 *>(gp->d);
+            const uint32_t flags = g.d[3];
+            if ((flags & kGeomValid) == 0) continue;
+            const uint32_t shape = flags & 0xFFu;
+            float t = 0.0f, al = 0.0f, be = 0.0f;
+            bool h = false;
+            if (shape == AVRT_SHAPE_MESH) {
+                // Leave the rest of this leaf for later and walk the
+                // instance's BLAS in mesh-local space (ToLocal).
+                if (i + 1 < count) st.push(make_leaf(slot + 1, count - i - 1));
+                st.push(kRefReturn);
+                const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];
+                ao = quat_rotate(qx, qy, qz, qw, o - f3(g.a[0], g.a[1], g.a[2]));
+                ad = quat_rotate(qx, qy, qz, qw, d);
+                const float s = g.a[3];
+                if (s != 1.0f) {
+                    ao = ao / s;
+                    ad = ad / s;
                 }
+                ainv = f3(1.0f / ad.x, 1.0f / ad.y, 1.0f / ad.z);
+                cur_obj = slot;
+                tri_base = __float_as_uint(g.c[1]);
+                in_blas = true;
+                ref = __float_as_uint(g.c[0]);
+                return true;
+            } else if (shape == AVRT_SHAPE_SPHERE) {
+                h = test_sphere(g, o, d, time, near, far, &t);
+            } else if (shape == AVRT_SHAPE_TRIANGLE) {
+                h = test_planar(g, true, o, d, near, far, &t, &al, &be);
+            } else if (shape == AVRT_SHAPE_QUAD) {
+                h = test_planar(g, false, o, d, near, far, &t, &al, &be);
             } else {
-                bool descend = false;
-                for (uint32_t i = 0; i < count; i++) {
-                    const uint32_t slot = first + i;
-                    const ObjGeom *gp = sc.geom + slot;
-                    ObjGeom g;
-                    *reinterpret_cast<float4 *>(g.a) = *reinterpret_cast<const float4 *>(gp->a);
-                    *reinterpret_cast<float4 *>(g.b) = *reinterpret_cast<const float4 *>(gp->b);
-                    *reinterpret_cast<float4 *>(g.c) = *reinterpret_cast<const float4 *>(gp->c);
-                    *reinterpret_cast<uint4 *>(g.d) = *reinterpret_cast<const uint4 *>(gp->d);
-                    const uint32_t flags = g.d[3];
-                    if ((flags & kGeomValid) == 0) continue;
-                    const uint32_t shape = flags & 0xFFu;
-                    float t = 0.0f, al = 0.0f, be = 0.0f;
-                    bool h = false;
-                    if (shape == AVRT_SHAPE_MESH) {
-                        // Leave the rest of this leaf for later and walk the
-                        // instance's BLAS in mesh-local space (ToLocal).
-                        if (i + 1 < count) st.push(make_leaf(slot + 1, count - i - 1));
-                        st.push(kRefReturn);
-                        const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];
-                        ao = quat_rotate(qx, qy, qz, qw, o - f3(g.a[0], g.a[1], g.a[2]));
-                        ad = quat_rotate(qx, qy, qz, qw, d);
-                        const float s = g.a[3];
-                        if (s != 1.0f) {
-                            ao = ao / s;
-                            ad = ad / s;
-                        }
-                        ainv = f3(1.0f / ad.x, 1.0f / ad.y, 1.0f / ad.z);
-                        cur_obj = slot;
-                        tri_base = __float_as_uint(g.c[1]);
-                        in_blas = true;
-                        ref = __float_as_uint(g.c[0]);
-                        descend = true;
-                        break;
-                    } else if (shape == AVRT_SHAPE_SPHERE) {
-                        h = test_sphere(g, o, d, time, near, far, &t);
-                    } else if (shape == AVRT_SHAPE_TRIANGLE) {
-                        h = test_planar(g, true, o, d, near, far, &t, &al, &be);
-                    } else if (shape == AVRT_SHAPE_QUAD) {
-                        h = test_planar(g, false, o, d, near, far, &t, &al, &be);
-                    } else {
-                        h = test_box(g, o, d, near, far, &t);
-                    }
-                    if (h) {
-                        far = t;
-                        found = true;
-                        hit.t = t;
-                        hit.obj = slot;
-                        hit.prim = ~0u;
-                        hit.alpha = al;
-                        hit.beta = be;
-                    }
-                }
-                if (descend) continue;
+                h = test_box(g, o, d, near, far, &t);
+            }
+            if (h) {
+                far = t;
+                hit.t = t;
+                hit.obj = slot;
+                hit.prim = ~0u;
+                hit.alpha = al;
+                hit.beta = be;
             }
         }
-        if (st.empty()) break;
-        ref = st.pop();
+        return next();
     }
-    if (node_visits) *node_visits = visits;
-    return found;
+};
+
+// Closest hit through TLAS and BLAS, run to completion.
+template <bool REF_ORDER, int K, int BLOCK, int SPILL>
+__device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, float time, float near, float far,
+                                              uint32_t *lds_stack, HitRec &hit, uint32_t *node_visits)
+{
+    Traversal<REF_ORDER, K, BLOCK, SPILL> tr;
+    if (tr.init(sc, o, d, time, near, far, lds_stack)) {
+        while (tr.step(sc)) {
+        }
+    }
+    hit = tr.hit;
+    if (node_visits) *node_visits = tr.steps;
+    return hit.obj != ~0u;
 }
 
 } // namespace dev

@@ -43,6 +43,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kStackLds = 16;
 constexpr int kStackSpill = 96;
+constexpr int kStepsPerRefill = 24;   // traversal steps between lane refills
+constexpr uint32_t kWaveLights = 4;   // NEE as shadow-ray wavefronts up to this many lights
 
 __constant__ float c_srgb[256];
 
@@ -284,21 +286,66 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t firs
     Lpath[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool REF_ORDER>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
-                                                   const float4 *__restrict__ rd, uint4 *__restrict__ hitA,
-                                                   float *__restrict__ hitB)
+// Closest-hit traversal of a ray queue by persistent lanes: a lane whose ray
+// is finished takes the next unclaimed ray (one atomic per wave), so a wave
+// does not idle behind its slowest ray. SHADOW rays are bounded by the light
+// point stored beside them: far = |lp - o| - 0.001 (CalculateDirectLight,
+// ray_trace.comp.hybrid.hlsl:1167-1179).
+template <bool REF_ORDER, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                  const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                  uint4 *__restrict__ hitA, float *__restrict__ hitB, uint32_t *work)
 {
     __shared__ uint32_t lds_stack[kStackLds * kBlock];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const float4 O = ro[k];
-    const float4 D = rd[k];
-    HitRec h;
-    trace_closest<REF_ORDER, kStackLds, kBlock, kStackSpill>(sc, f3(O.x, O.y, O.z), f3(D.x, D.y, D.z), O.w, 0.001f,
-                                                             1000000.0f, lds_stack, h, nullptr);
-    hitA[k] = make_uint4(__float_as_uint(h.t), h.obj, h.prim, __float_as_uint(h.alpha));
-    hitB[k] = h.beta;
+    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
+    uint32_t idx = 0;
+    bool have = false, done = false;
+    while (true) {
+        const bool need = !have && !done;
+        const uint64_t m = __ballot(need);
+        if (m != 0ull) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            uint32_t base = 0;
+            if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(work, static_cast<uint32_t>(__popcll(m)));
+            base = __shfl(base, leader);
+            if (need) {
+                idx = base + rank;
+                if (idx < n) {
+                    const float4 O = ro[idx];
+                    const float4 D = rd[idx];
+                    const F3 o = f3(O.x, O.y, O.z);
+                    float far = 1000000.0f;
+                    if (SHADOW) {
+                        const float4 L = rlp[idx];
+                        far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
+                    }
+                    have = tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack);
+                    if (!have) {
+                        hitA[idx] = make_uint4(__float_as_uint(far), ~0u, ~0u, 0u);
+                        hitB[idx] = 0.0f;
+                    }
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0ull) {
+            if (__ballot(!done) == 0ull) break;
+            continue;
+        }
+        if (have) {
+            for (int it = 0; it < kStepsPerRefill; it++) {
+                if (!tr.step(sc)) {
+                    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+                    hitB[idx] = tr.hit.beta;
+                    have = false;
+                    break;
+                }
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)
@@ -311,12 +358,34 @@ __device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)
 }
 
 // MODE 0 = path_trace.comp, MODE 1 = ray_trace.comp.hybrid.
-template <int MODE>
+struct ShadowRays {
+    float4 *o;   // origin.xyz, time
+    float4 *d;   // direction.xyz, contribution slot (bits)
+    float4 *lp;  // sampled light point.xyz, glass steps taken (bits)
+};
+
+struct NeeBufs {
+    float4 *tp;      // per record: throughput at the hit.xyz, path id
+    float4 *contrib; // per (record, light): unoccluded contribution
+};
+
+// MODE 0 = path_trace.comp, MODE 1 = ray_trace.comp.hybrid. WAVE_NEE: sample
+// the lights here and emit shadow rays for k_trace (up to kWaveLights lights);
+// otherwise store a record that k_shadow resolves in one kernel.
+template <int MODE, bool WAVE_NEE>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, uint32_t n, uint32_t bounce,
                                                   PathBufs in, const uint4 *__restrict__ hitA,
                                                   const float *__restrict__ hitB, PathBufs out, ShadowBufs sh,
-                                                  float4 *Lpath, uint32_t *counters)
+                                                  ShadowRays sr, NeeBufs nee, float4 *Lpath, uint32_t *counters)
 {
+    F3 lpt[kWaveLights], ldir[kWaveLights], lc[kWaveLights];
+    bool lray[kWaveLights];
+    for (uint32_t i = 0; i < kWaveLights; i++) {
+        lray[i] = false;
+        lpt[i] = f3(0, 0, 0);
+        ldir[i] = f3(0, 0, 0);
+        lc[i] = f3(0, 0, 0);
+    }
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     bool cont = false;
     bool want_shadow = false;
@@ -380,7 +449,64 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, u
                         albedo = salb;
                         tp_at_hit = tp;
                         rng_at_nee = rng.s;
-                        rng.s = rng.s * fp.jumpA + fp.jumpC;
+                        if (WAVE_NEE) {
+                            // SampleLight and the early-outs of CalculateDirectLight
+                            // (:1130-1149); the occlusion test is deferred to k_trace.
+                            for (uint32_t i = 0; i < kWaveLights; i++) {
+                                if (i >= fp.num_lights) break;
+                                const AvrtObject l = sc.objects[sc.lights[i]];
+                                const F3 LP = f3(l.x, l.y, l.z);
+                                F3 lp = LP;
+                                float solid = 0.0f;
+                                if (l.shapeType == AVRT_SHAPE_QUAD) {
+                                    const F3 lu = f3(l.u_x, l.u_y, l.u_z), lv = f3(l.v_x, l.v_y, l.v_z);
+                                    const float r1 = rng.next();
+                                    const float r2 = rng.next();
+                                    lp = LP + r1 * lu + r2 * lv;
+                                    const F3 cuv = cross(lu, lv);
+                                    const float area = length(cuv);
+                                    const F3 lnrm = cuv / fmaxf(area, 1e-8f);
+                                    F3 ts = s.pos - lp;
+                                    const float d2 = dot(ts, ts);
+                                    ts = normalize(ts);
+                                    const float cl = fabsf(dot(lnrm, ts));
+                                    solid = (cl * area) / fmaxf(d2, 1e-8f);
+                                } else if (l.shapeType == AVRT_SHAPE_SPHERE) {
+                                    const F3 tl = LP - s.pos;
+                                    const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+                                    const float r2 = l.radius * l.radius;
+                                    if (d2 <= r2) {
+                                        solid = 4.0f * 3.14159265f;
+                                    } else {
+                                        const float ctm = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+                                        solid = 2.0f * 3.14159265f * (1.0f - ctm);
+                                    }
+                                } else {
+                                    const F3 tl = LP - s.pos;
+                                    const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+                                    const float area =
+                                        4.0f * (l.half_x * l.half_y + l.half_y * l.half_z + l.half_z * l.half_x) / 6.0f;
+                                    solid = area / d2;
+                                }
+                                if (solid > 0.0f) {
+                                    F3 dir = lp - s.pos;
+                                    const float dist = length(dir);
+                                    if (dist > 1e-6f) {
+                                        dir = dir / dist;
+                                        const float ct = dot(s.normal, dir);
+                                        if (ct > 0.0f) {
+                                            const F3 radiance = l.emission * f3(l.r, l.g, l.b);
+                                            lc[i] = salb * radiance * ct * solid;
+                                            lray[i] = true;
+                                            lpt[i] = lp;
+                                            ldir[i] = dir;
+                                        }
+                                    }
+                                }
+                            }
+                        } else {
+                            rng.s = rng.s * fp.jumpA + fp.jumpC;
+                        }
                     }
                     F3 sc_dir;
                     if (get_bounce(ob, d, s, rng, &sc_dir)) {
@@ -413,7 +539,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, u
         out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(pid));
         out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(nrng));
     }
-    if (MODE == 1) {
+    if (MODE == 1 && !WAVE_NEE) {
         const uint32_t ss = wave_append(want_shadow, &counters[1]);
         if (want_shadow) {
             sh.p[ss] = make_float4(s.pos.x, s.pos.y, s.pos.z, time);
@@ -422,6 +548,87 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, u
             sh.w[ss] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, 0.0f);
         }
     }
+    if (MODE == 1 && WAVE_NEE) {
+        const uint32_t rec = wave_append(want_shadow, &counters[1]);
+        const uint32_t nl = fp.num_lights;
+        if (want_shadow) {
+            nee.tp[rec] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, __uint_as_float(pid));
+            for (uint32_t i = 0; i < kWaveLights; i++)
+                if (i < nl) nee.contrib[rec * nl + i] = make_float4(lc[i].x, lc[i].y, lc[i].z, 0.0f);
+        }
+        const F3 so = s.pos + (s.normal * 0.001f);
+        for (uint32_t i = 0; i < kWaveLights; i++) {
+            const bool emit = want_shadow && lray[i];
+            const uint32_t r = wave_append(emit, &counters[2]);
+            if (emit) {
+                sr.o[r] = make_float4(so.x, so.y, so.z, time);
+                sr.d[r] = make_float4(ldir[i].x, ldir[i].y, ldir[i].z, __uint_as_float(rec * nl + i));
+                sr.lp[r] = make_float4(lpt[i].x, lpt[i].y, lpt[i].z, __uint_as_float(0u));
+            }
+        }
+    }
+}
+
+// Classification of traced shadow rays (the loop body of CalculateDirectLight,
+// ray_trace.comp.hybrid.hlsl:1167-1191): no hit before the light, or the light
+// itself -> unoccluded; glass -> step through (at most 4 queries per light);
+// anything else -> occluded, its contribution dropped.
+__global__ __launch_bounds__(kBlock) void k_shadow_resolve(SceneDev sc, uint32_t n, ShadowRays in,
+                                                           const uint4 *__restrict__ hitA, ShadowRays out,
+                                                           float4 *contrib, uint32_t *counter)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool more = false;
+    F3 no = f3(0, 0, 0), dir = f3(0, 0, 0), lp = f3(0, 0, 0);
+    float time = 0.0f;
+    uint32_t slot = 0, step = 0;
+    if (k < n) {
+        const float4 O = in.o[k], D = in.d[k], L = in.lp[k];
+        const uint4 H = hitA[k];
+        const F3 so = f3(O.x, O.y, O.z);
+        dir = f3(D.x, D.y, D.z);
+        lp = f3(L.x, L.y, L.z);
+        time = O.w;
+        slot = __float_as_uint(D.w);
+        step = __float_as_uint(L.w);
+        const float t = __uint_as_float(H.x);
+        const float bound = length(lp - so) - 0.001f;
+        if (H.y != ~0u && !(t >= bound)) {
+            const uint32_t ct = sc.objects[H.y].colorType;
+            if (ct != AVRT_DIFFUSE_LIGHT) {
+                if (ct != AVRT_DIELECTRIC) {
+                    contrib[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                } else if (step + 1u < 4u) {
+                    more = true;
+                    no = (so + dir * t) + (dir * 0.001f);
+                    step = step + 1u;
+                }
+            }
+        }
+    }
+    const uint32_t r = wave_append(more, counter);
+    if (more) {
+        out.o[r] = make_float4(no.x, no.y, no.z, time);
+        out.d[r] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
+        out.lp[r] = make_float4(lp.x, lp.y, lp.z, __uint_as_float(step));
+    }
+}
+
+// Sum of the light contributions in light order, clamped to 8 and weighted by
+// the throughput (ColorRay, ray_trace.comp.hybrid.hlsl:1264-1274).
+__global__ __launch_bounds__(kBlock) void k_nee_finalize(uint32_t n, uint32_t nl, NeeBufs nee, float4 *Lpath)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    F3 sum = f3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < nl; i++) {
+        const float4 c = nee.contrib[r * nl + i];
+        sum = sum + f3(c.x, c.y, c.z);
+    }
+    const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
+    if (luma > 8.0f) sum = sum * (8.0f / luma);
+    const float4 T = nee.tp[r];
+    add_radiance(Lpath, __float_as_uint(T.w), f3(T.x, T.y, T.z) * sum);
 }
 
 template <bool REF_ORDER>
@@ -732,6 +939,12 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
         const double v = i / 255.0;
         lut[i] = static_cast<float>(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (c->cu_count <= 0) c->cu_count = 256;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
+    c->trace_blocks_per_cu = nb;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
@@ -752,7 +965,8 @@ int avrt_destroy(avrt_ctx *c)
                       &c->sd[0], &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn,
                       &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
                       &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
-                      &c->rb.scratch, &c->rb.info};
+                      &c->rb.scratch, &c->rb.info, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
+                      &c->srl[1], &c->nee_tp, &c->nee_contrib};
     for (DevBuf *b : bufs) release(*b);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
         if (e) (void)hipEventDestroy(e);
@@ -938,24 +1152,39 @@ static void rng_jump(uint32_t n, uint32_t *ja, uint32_t *jc)
 
 static inline uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 
-static int ensure_paths(avrt_ctx *c, size_t n)
+static int ensure_paths(avrt_ctx *c, size_t n, uint32_t nlights)
 {
-    if (n <= c->cap_paths) return AVRT_OK;
     int rc = 0;
-    for (int i = 0; i < 2; i++) {
-        if ((rc = ensure(c, c->so[i], n * 16))) return rc;
-        if ((rc = ensure(c, c->sd[i], n * 16))) return rc;
-        if ((rc = ensure(c, c->st[i], n * 16))) return rc;
+    if (n > c->cap_paths) {
+        for (int i = 0; i < 2; i++) {
+            if ((rc = ensure(c, c->so[i], n * 16))) return rc;
+            if ((rc = ensure(c, c->sd[i], n * 16))) return rc;
+            if ((rc = ensure(c, c->st[i], n * 16))) return rc;
+        }
+        if ((rc = ensure(c, c->hitA, n * 16))) return rc;
+        if ((rc = ensure(c, c->hitB, n * 4))) return rc;
+        if ((rc = ensure(c, c->Lpath, n * 16))) return rc;
+        if ((rc = ensure(c, c->shp, n * 16))) return rc;
+        if ((rc = ensure(c, c->shn, n * 16))) return rc;
+        if ((rc = ensure(c, c->sha, n * 16))) return rc;
+        if ((rc = ensure(c, c->shw, n * 16))) return rc;
+        if ((rc = ensure(c, c->nee_tp, n * 16))) return rc;
+        if ((rc = ensure(c, c->counters, 64))) return rc;
+        c->cap_paths = n;
     }
-    if ((rc = ensure(c, c->hitA, n * 16))) return rc;
-    if ((rc = ensure(c, c->hitB, n * 4))) return rc;
-    if ((rc = ensure(c, c->Lpath, n * 16))) return rc;
-    if ((rc = ensure(c, c->shp, n * 16))) return rc;
-    if ((rc = ensure(c, c->shn, n * 16))) return rc;
-    if ((rc = ensure(c, c->sha, n * 16))) return rc;
-    if ((rc = ensure(c, c->shw, n * 16))) return rc;
-    if ((rc = ensure(c, c->counters, 64))) return rc;
-    c->cap_paths = n;
+    const size_t ns = n * (nlights <= kWaveLights ? nlights : 0);
+    if (ns > c->cap_shadow) {
+        for (int i = 0; i < 2; i++) {
+            if ((rc = ensure(c, c->sro[i], ns * 16))) return rc;
+            if ((rc = ensure(c, c->srd[i], ns * 16))) return rc;
+            if ((rc = ensure(c, c->srl[i], ns * 16))) return rc;
+        }
+        if ((rc = ensure(c, c->nee_contrib, ns * 16))) return rc;
+        // the shadow hits reuse hitA/hitB, which must then hold ns entries
+        if ((rc = ensure(c, c->hitA, ns * 16))) return rc;
+        if ((rc = ensure(c, c->hitB, ns * 4))) return rc;
+        c->cap_shadow = ns;
+    }
     return AVRT_OK;
 }
 
@@ -992,6 +1221,40 @@ static int validate_config(avrt_ctx *c, const AvrtConfig *cfg)
     return AVRT_OK;
 }
 
+// Grid of the persistent traversal kernels: as many workgroups as can be
+// resident at once (more only queue behind them).
+static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
+{
+    const uint32_t resident = static_cast<uint32_t>(c->cu_count * c->trace_blocks_per_cu);
+    return std::max(1u, std::min(blocks_for(n), resident));
+}
+
+static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n, const float4 *ro, const float4 *rd,
+                        const float4 *rlp)
+{
+    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + 8;
+    HIPX(c, hipMemsetAsync(work, 0, 4, c->stream));
+    uint4 *hA = static_cast<uint4 *>(c->hitA.p);
+    float *hB = static_cast<float *>(c->hitB.p);
+    const dim3 grid(trace_grid(c, n)), block(kBlock);
+    if (c->ref_order) {
+        if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+        else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+    } else {
+        if (shadow) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+        else hipLaunchKernelGGL((k_trace<false, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+    }
+    HIPX(c, hipGetLastError());
+    return AVRT_OK;
+}
+
+static int read_counters(avrt_ctx *c, uint32_t *host4)
+{
+    HIPX(c, hipMemcpyAsync(host4, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
 // Launches the bounce loop for `ns` sample frames starting at sample s0.
 template <int MODE>
 static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns)
@@ -1005,6 +1268,11 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
         buf[i] = PathBufs{static_cast<float4 *>(c->so[i].p), static_cast<float4 *>(c->sd[i].p), static_cast<float4 *>(c->st[i].p)};
     ShadowBufs sh{static_cast<float4 *>(c->shp.p), static_cast<float4 *>(c->shn.p), static_cast<float4 *>(c->sha.p),
                   static_cast<float4 *>(c->shw.p)};
+    ShadowRays sr[2];
+    for (int i = 0; i < 2; i++)
+        sr[i] = ShadowRays{static_cast<float4 *>(c->sro[i].p), static_cast<float4 *>(c->srd[i].p), static_cast<float4 *>(c->srl[i].p)};
+    NeeBufs nee{static_cast<float4 *>(c->nee_tp.p), static_cast<float4 *>(c->nee_contrib.p)};
+    const bool wave_nee = (MODE == 1) && fp.num_lights > 0 && fp.num_lights <= kWaveLights;
     float4 *Lpath = static_cast<float4 *>(c->Lpath.p);
     HIPX(c, hipEventRecord(c->ev_a, c->stream));
     hipLaunchKernelGGL(k_raygen, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, fp, s0, n, buf[0], Lpath);
@@ -1019,42 +1287,63 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
         else c->stats.rays_bounce += n;
         HIPX(c, hipMemsetAsync(counters, 0, 16, c->stream));
         HIPX(c, hipEventRecord(c->ev_a, c->stream));
-        if (c->ref_order)
-            hipLaunchKernelGGL(k_extend<true>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, n, buf[cur].o, buf[cur].d,
-                               static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
-        else
-            hipLaunchKernelGGL(k_extend<false>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, n, buf[cur].o, buf[cur].d,
-                               static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
-        HIPX(c, hipGetLastError());
+        int rc = launch_trace(c, sc, false, n, buf[cur].o, buf[cur].d, nullptr);
+        if (rc) return rc;
         HIPX(c, hipEventRecord(c->ev_b, c->stream));
         c->stats.launches_extend++;
-        hipLaunchKernelGGL(k_shade<MODE>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
-                           static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
-                           Lpath, counters);
+        if (wave_nee)
+            hipLaunchKernelGGL((k_shade<MODE, true>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
+                               static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
+                               sr[0], nee, Lpath, counters);
+        else
+            hipLaunchKernelGGL((k_shade<MODE, false>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
+                               static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
+                               sr[0], nee, Lpath, counters);
         HIPX(c, hipGetLastError());
         HIPX(c, hipEventRecord(c->ev1, c->stream));
-        uint32_t host_counts[4] = {0, 0, 0, 0};
-        HIPX(c, hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, c->stream));
-        HIPX(c, hipStreamSynchronize(c->stream));
+        uint32_t hc[4] = {0, 0, 0, 0};
+        if ((rc = read_counters(c, hc))) return rc;
         c->stats.ms_extend += elapsed(c->ev_a, c->ev_b);
         c->stats.ms_shade += elapsed(c->ev_b, c->ev1);
-        const uint32_t nshadow = host_counts[1];
-        if (MODE == 1 && nshadow > 0) {
+        const uint32_t next_n = hc[0], nrec = hc[1];
+        if (MODE == 1 && wave_nee && nrec > 0) {
+            uint32_t nsr = hc[2];
+            int q = 0;
             HIPX(c, hipEventRecord(c->ev_a, c->stream));
-            if (c->ref_order)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(blocks_for(nshadow)), dim3(kBlock), 0, c->stream, sc, fp, nshadow, sh,
-                                   Lpath, counters);
-            else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(blocks_for(nshadow)), dim3(kBlock), 0, c->stream, sc, fp, nshadow, sh,
-                                   Lpath, counters);
+            for (int step = 0; step < 4 && nsr > 0; step++) {
+                c->stats.rays_shadow += nsr;
+                if ((rc = launch_trace(c, sc, true, nsr, sr[q].o, sr[q].d, sr[q].lp))) return rc;
+                HIPX(c, hipMemsetAsync(counters + 3, 0, 4, c->stream));
+                hipLaunchKernelGGL(k_shadow_resolve, dim3(blocks_for(nsr)), dim3(kBlock), 0, c->stream, sc, nsr, sr[q],
+                                   static_cast<const uint4 *>(c->hitA.p), sr[q ^ 1], nee.contrib, counters + 3);
+                HIPX(c, hipGetLastError());
+                uint32_t h2[4];
+                if ((rc = read_counters(c, h2))) return rc;
+                nsr = h2[3];
+                q ^= 1;
+            }
+            hipLaunchKernelGGL(k_nee_finalize, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, nrec, fp.num_lights, nee,
+                               Lpath);
             HIPX(c, hipGetLastError());
             HIPX(c, hipEventRecord(c->ev_b, c->stream));
-            HIPX(c, hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, c->stream));
-            HIPX(c, hipStreamSynchronize(c->stream));
+            HIPX(c, hipEventSynchronize(c->ev_b));
             c->stats.ms_shadow += elapsed(c->ev_a, c->ev_b);
-            c->stats.rays_shadow += host_counts[2];
+        } else if (MODE == 1 && nrec > 0) {
+            HIPX(c, hipEventRecord(c->ev_a, c->stream));
+            if (c->ref_order)
+                hipLaunchKernelGGL(k_shadow<true>, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, sc, fp, nrec, sh, Lpath,
+                                   counters);
+            else
+                hipLaunchKernelGGL(k_shadow<false>, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, sc, fp, nrec, sh, Lpath,
+                                   counters);
+            HIPX(c, hipGetLastError());
+            HIPX(c, hipEventRecord(c->ev_b, c->stream));
+            uint32_t h2[4];
+            if ((rc = read_counters(c, h2))) return rc;
+            c->stats.ms_shadow += elapsed(c->ev_a, c->ev_b);
+            c->stats.rays_shadow += h2[2];
         }
-        n = host_counts[0];
+        n = next_n;
         cur ^= 1;
     }
     HIPX(c, hipEventRecord(c->ev_a, c->stream));
@@ -1113,7 +1402,7 @@ int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
     size_t cap = c->opts.max_paths ? c->opts.max_paths : static_cast<size_t>(P) * S;
     if (cap < P) cap = P;
     const uint32_t per_chunk = static_cast<uint32_t>(std::min<size_t>(S, cap / P));
-    rc = ensure_paths(c, static_cast<size_t>(P) * per_chunk);
+    rc = ensure_paths(c, static_cast<size_t>(P) * per_chunk, c->nlights);
     if (rc) return rc;
     for (uint32_t s0 = 0; s0 < S; s0 += per_chunk) {
         const uint32_t ns = std::min(per_chunk, S - s0);
@@ -1167,20 +1456,15 @@ int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object,
     FrameParams fp;
     frame_params(c, cfg, fp);
     const uint32_t P = cfg->width * cfg->height;
-    rc = ensure_paths(c, P);
+    rc = ensure_paths(c, P, 0);
     if (rc) return rc;
     PathBufs buf{static_cast<float4 *>(c->so[0].p), static_cast<float4 *>(c->sd[0].p), static_cast<float4 *>(c->st[0].p)};
     hipLaunchKernelGGL(k_raygen, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, 0u, P, buf,
                        static_cast<float4 *>(c->Lpath.p));
     HIPX(c, hipGetLastError());
     const SceneDev sc = scene_dev(c);
-    if (c->ref_order)
-        hipLaunchKernelGGL(k_extend<true>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, sc, P, buf.o, buf.d,
-                           static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
-    else
-        hipLaunchKernelGGL(k_extend<false>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, sc, P, buf.o, buf.d,
-                           static_cast<uint4 *>(c->hitA.p), static_cast<float *>(c->hitB.p));
-    HIPX(c, hipGetLastError());
+    rc = launch_trace(c, sc, false, P, buf.o, buf.d, nullptr);
+    if (rc) return rc;
     std::vector<uint4> h(P);
     HIPX(c, hipMemcpyAsync(h.data(), c->hitA.p, sizeof(uint4) * P, hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));

@@ -144,6 +144,36 @@ def test_render_reference_order_is_bit_exact(c3):
     _compare("instanced-refordered", img, ref, min_exact=0.9999)
 
 
+def _with_lights(spec, count):
+    """The icosphere scene with `count` lights: the quad plus extra emissive
+    spheres and quads around it."""
+    extra = []
+    for k in range(count - 1):
+        a = 2 * np.pi * k / max(1, count - 1)
+        if k % 2 == 0:
+            extra.append(scenes._obj(abi.SPHERE, abi.DIFFUSE_LIGHT, (3 * np.cos(a), 2.5, 3 * np.sin(a)), 0.4,
+                                     albedo=(1.0, 0.8, 0.6), emission=4.0))
+        else:
+            extra.append(scenes._obj(abi.QUAD, abi.DIFFUSE_LIGHT, (3 * np.cos(a), 3.0, 3 * np.sin(a)), u=(0.8, 0, 0),
+                                     v=(0, 0, 0.8), albedo=(0.6, 0.8, 1.0), emission=6.0))
+    spec.objects = np.concatenate([np.array(extra, abi.OBJECT_DTYPE), spec.objects]) if extra else spec.objects
+    return scenes.pack(spec)
+
+
+@pytest.mark.parametrize("nlights", [3, 6])
+def test_render_several_lights(nlights):
+    """3 lights: shadow-ray wavefronts with several lights per path; 6 lights:
+    the single-kernel NEE path."""
+    pk = _with_lights(scenes.icosphere_tris(level=3), nlights)
+    assert len(pk.lights) == nlights
+    cfg = small_cfg(pk.config, 160, 90, 2)
+    say(f"render {nlights} lights")
+    img, _, stats = render_gpu(pk, cfg, abi.MODE_RAY_TRACE)
+    ref, _, _, _ = oracle.render(pk, cfg, abi.MODE_RAY_TRACE)
+    _compare(f"{nlights} lights", img, ref)
+    assert stats["rays_shadow"] > 0
+
+
 def test_fp16_accumulation_matches_oracle(c2):
     cfg = small_cfg(c2.config, 160, 90, 3)
     say("render icosphere_tris fp16 accumulation")
