@@ -37,6 +37,7 @@ struct SceneRebuild {
     DevBuf vel;              // float[6m]
     DevBuf bounds;           // AvrtAABB[n]
     DevBuf nodes;            // AvrtBVHNode[2n]
+    DevBuf depth;            // uint32[2n]: depth of each node
     DevBuf order;            // uint32[n]
     DevBuf scratch;          // builder work arrays
     DevBuf info;             // uint32[8]: node count, error flags
@@ -57,7 +58,7 @@ struct avrt_ctx {
     std::vector<AvrtVertex> h_verts;
     std::vector<uint32_t> h_indices;
     std::vector<AvrtBVHNode> h_blas;
-    std::vector<avrt::Node2> h_blas2;
+    std::vector<avrt::Node4> h_blas2;
     std::vector<avrt::TriPre> h_tris;
     std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
     std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase

@@ -124,20 +124,24 @@ __device__ __forceinline__ F3 rotate_y(F3 v, float s, float c) { return f3(c * v
 
 // ---------------------------------------------------------------------------
 // Traversal stack: a ring of K entries per lane in LDS holding the top of the
-// stack, older entries spilled to a per-lane scratch array. The LDS ring is
-// laid out [slot][lane] so a wave's accesses hit 64 consecutive dwords.
-// ---------------------------------------------------------------------------
+// stack, older entries spilled to a per-lane array of SPILL entries in private
+// memory. The LDS ring is laid out [slot][lane] so a wave's accesses hit 64
+// consecutive dwords. The spill array is owned by the caller and reached by
+// pointer, so that only overflowing pushes and the pops that follow them touch
+// private memory; everything else in the traversal state stays in registers.
+// A push beyond K + SPILL entries sets `overflow` and drops the entry.
 template <int K, int BLOCK, int SPILL>
 struct ShortStack {
-    uint32_t *col; // this lane's column: lds + threadIdx.x
-    uint32_t spill[SPILL > 0 ? SPILL : 1];
+    uint32_t *col;   // this lane's column: lds + threadIdx.x
+    uint32_t *spill; // SPILL entries
     int sp;
     int lo; // lowest entry still held in LDS
     bool overflow;
 
-    __device__ __forceinline__ void init(uint32_t *lds_base)
+    __device__ __forceinline__ void init(uint32_t *lds_base, uint32_t *spill_mem)
     {
         col = lds_base + threadIdx.x;
+        spill = spill_mem;
         sp = 0;
         lo = 0;
         overflow = false;
@@ -145,11 +149,6 @@ struct ShortStack {
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(uint32_t v)
     {
-        if (K == 0) {
-            if (sp < SPILL) spill[sp++] = v;
-            else overflow = true;
-            return;
-        }
         if (sp - lo == K) {
             if (lo >= SPILL) {
                 overflow = true;
@@ -164,7 +163,6 @@ struct ShortStack {
     __device__ __forceinline__ uint32_t pop()
     {
         sp--;
-        if (K == 0) return spill[sp];
         if (sp < lo) {
             lo = sp;
             return spill[sp];
@@ -172,7 +170,6 @@ struct ShortStack {
         return col[(sp & (K - 1)) * BLOCK];
     }
 };
-
 // Slab test of one box (IntersectAABB): t0=(min-o)*inv, t1=(max-o)*inv,
 // tmin = max(max(ts.x,ts.y), max(ts.z,near)), tmax likewise with far.
 __device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy, float loz, float hiz, F3 o, F3 inv,
@@ -200,9 +197,9 @@ struct HitRec {
 };
 
 struct SceneDev {
-    const Node2 *tlas;       // super-root at 0 (null when the scene is empty)
+    const Node4 *tlas;       // super-root at 0 (null when the scene is empty)
     const ObjGeom *geom;     // per TLAS slot
-    const Node2 *blas;       // all BLAS trees, each with its super-root
+    const Node4 *blas;       // all BLAS trees, each with its super-root
     const TriPre *tris;      // per mesh triangle, index base/3 + slot
     const AvrtObject *objects;
     const AvrtVertex *verts;
@@ -320,7 +317,7 @@ struct Traversal {
     ShortStack<K, BLOCK, SPILL> st;
 
     __device__ __forceinline__ bool init(const SceneDev &sc, F3 o_, F3 d_, float time_, float near_, float far_,
-                                         uint32_t *lds_stack)
+                                         uint32_t *lds_stack, uint32_t *spill_mem)
     {
         o = o_;
         d = d_;
@@ -336,7 +333,7 @@ struct Traversal {
         ao = o;
         ad = d;
         ainv = winv;
-        st.init(lds_stack);
+        st.init(lds_stack, spill_mem);
         in_blas = false;
         cur_obj = 0;
         tri_base = 0;
@@ -366,29 +363,65 @@ struct Traversal {
             return next();
         }
         if ((ref & kLeafFlag) == 0) {
-            const Node2 *np = (in_blas ? sc.blas : sc.tlas) + ref;
-            const float4 X = *reinterpret_cast<const float4 *>(np->bx);
-            const float4 Y = *reinterpret_cast<const float4 *>(np->by);
-            const float4 Z = *reinterpret_cast<const float4 *>(np->bz);
+            const Node4 *np = (in_blas ? sc.blas : sc.tlas) + ref;
+            const float4 LX = *reinterpret_cast<const float4 *>(np->lox);
+            const float4 HX = *reinterpret_cast<const float4 *>(np->hix);
+            const float4 LY = *reinterpret_cast<const float4 *>(np->loy);
+            const float4 HY = *reinterpret_cast<const float4 *>(np->hiy);
+            const float4 LZ = *reinterpret_cast<const float4 *>(np->loz);
+            const float4 HZ = *reinterpret_cast<const float4 *>(np->hiz);
             const uint4 R = *reinterpret_cast<const uint4 *>(np->ref);
-            float e0, e1;
-            const bool h0 = slab(X.x, X.z, Y.x, Y.z, Z.x, Z.z, ao, ainv, near, far, &e0) && R.x != kRefInvalid;
-            const bool h1 = slab(X.y, X.w, Y.y, Y.w, Z.y, Z.w, ao, ainv, near, far, &e1) && R.y != kRefInvalid;
-            if (h0 && h1) {
-                const bool first1 = REF_ORDER ? true : (e1 < e0);
-                st.push(first1 ? R.x : R.y);
-                ref = first1 ? R.y : R.x;
+            float t0, t1, t2, t3;
+            const bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, ao, ainv, near, far, &t0) && R.x != kRefInvalid;
+            const bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, ao, ainv, near, far, &t1) && R.y != kRefInvalid;
+            const bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, ao, ainv, near, far, &t2) && R.z != kRefInvalid;
+            const bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, ao, ainv, near, far, &t3) && R.w != kRefInvalid;
+            if (REF_ORDER) {
+                // right to left: the highest-index child next, the others
+                // pushed left to right so they pop right to left
+                uint32_t nxt = kRefInvalid;
+                if (h0) nxt = R.x;
+                if (h1) {
+                    if (nxt != kRefInvalid) st.push(nxt);
+                    nxt = R.y;
+                }
+                if (h2) {
+                    if (nxt != kRefInvalid) st.push(nxt);
+                    nxt = R.z;
+                }
+                if (h3) {
+                    if (nxt != kRefInvalid) st.push(nxt);
+                    nxt = R.w;
+                }
+                if (nxt == kRefInvalid) return next();
+                ref = nxt;
                 return true;
             }
-            if (h0) {
-                ref = R.x;
-                return true;
-            }
-            if (h1) {
-                ref = R.y;
-                return true;
-            }
-            return next();
+            // nearest child next, the others pushed farthest first
+            const float kMiss = 3.0e38f;
+            float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
+            uint32_t r0 = R.x, r1 = R.y, r2 = R.z, r3 = R.w;
+#define AVRT_CSWAP(ka, ra, kb, rb)                                                                                  \
+    if (kb < ka) {                                                                                                 \
+        const float tk = ka;                                                                                       \
+        ka = kb;                                                                                                   \
+        kb = tk;                                                                                                   \
+        const uint32_t tr = ra;                                                                                    \
+        ra = rb;                                                                                                   \
+        rb = tr;                                                                                                   \
+    }
+            AVRT_CSWAP(k0, r0, k1, r1)
+            AVRT_CSWAP(k2, r2, k3, r3)
+            AVRT_CSWAP(k0, r0, k2, r2)
+            AVRT_CSWAP(k1, r1, k3, r3)
+            AVRT_CSWAP(k1, r1, k2, r2)
+#undef AVRT_CSWAP
+            if (!(k0 < kMiss)) return next();
+            if (k3 < kMiss) st.push(r3);
+            if (k2 < kMiss) st.push(r2);
+            if (k1 < kMiss) st.push(r1);
+            ref = r0;
+            return true;
         }
         const uint32_t first = ref & 0x00FFFFFFu;
         const uint32_t count = (ref >> 24) & 0x7Fu;
@@ -479,8 +512,9 @@ template <bool REF_ORDER, int K, int BLOCK, int SPILL>
 __device__ __forceinline__ bool trace_closest(const SceneDev &sc, F3 o, F3 d, float time, float near, float far,
                                               uint32_t *lds_stack, HitRec &hit, uint32_t *node_visits)
 {
+    uint32_t spill_mem[SPILL];
     Traversal<REF_ORDER, K, BLOCK, SPILL> tr;
-    if (tr.init(sc, o, d, time, near, far, lds_stack)) {
+    if (tr.init(sc, o, d, time, near, far, lds_stack, spill_mem)) {
         while (tr.step(sc)) {
         }
     }

@@ -4,13 +4,17 @@
 // MeshInstance_GPU) are accepted unchanged at the C boundary and converted on
 // upload into layouts that suit CDNA4 traversal:
 //
-//   Node2   64 B  one interior node of a binary BVH holding BOTH children's
-//                 boxes, so one node fetch (four dwordx4 loads, half a 128 B
-//                 line) tests two boxes. Child references are tagged:
-//                 interior = node index; leaf = LEAF | count << 24 | first.
+//   Node4  128 B  a 4-wide node: one 128 B line holds the boxes of up to four
+//                 children, so one fetch tests four boxes. It is the binary
+//                 node j of the reference tree with its children expanded one
+//                 level: each child that is interior contributes its own two
+//                 children, a leaf child itself (2 to 4 children, in the
+//                 binary tree's left-to-right order; visiting them right to
+//                 left is the reference's order). Node4 exist for the binary
+//                 interior nodes at even depth. Child references are tagged:
+//                 interior = Node4 index; leaf = LEAF | count << 24 | first.
 //                 Each tree starts with a "super-root" whose child 0 is the
-//                 reference root (so the root box is tested, bvh.h:275) and
-//                 whose child 1 is empty.
+//                 reference root (so the root box is tested, bvh.h:275).
 //   TriPre  64 B  per mesh triangle in BLAS slot order: p0, e1, e2 and
 //                 w = n / |n|^2 plus dot(n, p0), exactly the quantities
 //                 HitMeshTriangle (ray_trace.comp.hybrid.hlsl:564-598)
@@ -39,15 +43,14 @@ inline constexpr uint32_t make_leaf(uint32_t first, uint32_t count)
 // A TLAS "instance entry" on the traversal stack: a leaf with count 0.
 inline constexpr uint32_t make_inst_entry(uint32_t slot) { return kLeafFlag | slot; }
 
-struct alignas(16) Node2 {
-    // x: (c0.min, c1.min, c0.max, c1.max), same for y and z.
-    float bx[4];
-    float by[4];
-    float bz[4];
-    uint32_t ref[2];
-    uint32_t pad[2];
+struct alignas(128) Node4 {
+    float lox[4], hix[4];
+    float loy[4], hiy[4];
+    float loz[4], hiz[4];
+    uint32_t ref[4];
+    uint32_t pad[4];
 };
-static_assert(sizeof(Node2) == 64, "Node2 is 64 bytes");
+static_assert(sizeof(Node4) == 128, "Node4 is one 128-byte line");
 
 struct alignas(16) TriPre {
     float p0[3];

@@ -286,6 +286,35 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t firs
     Lpath[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Closest-hit traversal of a ray queue, one ray per lane. SHADOW rays are
+// bounded by the light point stored beside them: far = |lp - o| - 0.001
+// (CalculateDirectLight, ray_trace.comp.hybrid.hlsl:1167-1179).
+template <bool REF_ORDER, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace_lane(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                       const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                       uint4 *__restrict__ hitA, float *__restrict__ hitB)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+    uint32_t spill_mem[kStackSpill];
+    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
+    const float4 O = ro[idx];
+    const float4 D = rd[idx];
+    const F3 o = f3(O.x, O.y, O.z);
+    float far = 1000000.0f;
+    if (SHADOW) {
+        const float4 L = rlp[idx];
+        far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
+    }
+    if (tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem)) {
+        while (tr.step(sc)) {
+        }
+    }
+    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+    hitB[idx] = tr.hit.beta;
+}
+
 // Closest-hit traversal of a ray queue by persistent lanes: a lane whose ray
 // is finished takes the next unclaimed ray (one atomic per wave), so a wave
 // does not idle behind its slowest ray. SHADOW rays are bounded by the light
@@ -297,6 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
                                                   uint4 *__restrict__ hitA, float *__restrict__ hitB, uint32_t *work)
 {
     __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    uint32_t spill_mem[kStackSpill];
     Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
     uint32_t idx = 0;
     bool have = false, done = false;
@@ -321,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
                         const float4 L = rlp[idx];
                         far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
                     }
-                    have = tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack);
+                    have = tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem);
                     if (!have) {
                         hitA[idx] = make_uint4(__float_as_uint(far), ~0u, ~0u, 0u);
                         hitB[idx] = 0.0f;
@@ -891,7 +921,7 @@ int avrt::upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes)
 
 int avrt::upload_mesh_tables(avrt_ctx *c)
 {
-    int rc = upload_vec(c, c->d_blas2, c->h_blas2.data(), sizeof(Node2) * c->h_blas2.size());
+    int rc = upload_vec(c, c->d_blas2, c->h_blas2.data(), sizeof(Node4) * c->h_blas2.size());
     if (rc) return rc;
     return upload_vec(c, c->d_tris, c->h_tris.data(), sizeof(TriPre) * c->h_tris.size());
 }
@@ -899,9 +929,9 @@ int avrt::upload_mesh_tables(avrt_ctx *c)
 static SceneDev scene_dev(const avrt_ctx *c)
 {
     SceneDev s{};
-    s.tlas = c->nobj ? static_cast<const Node2 *>(c->d_tlas2.p) : nullptr;
+    s.tlas = c->nobj ? static_cast<const Node4 *>(c->d_tlas2.p) : nullptr;
     s.geom = static_cast<const ObjGeom *>(c->d_geom.p);
-    s.blas = static_cast<const Node2 *>(c->d_blas2.p);
+    s.blas = static_cast<const Node4 *>(c->d_blas2.p);
     s.tris = static_cast<const TriPre *>(c->d_tris.p);
     s.objects = static_cast<const AvrtObject *>(c->d_objects.p);
     s.verts = static_cast<const AvrtVertex *>(c->d_verts.p);
@@ -965,7 +995,7 @@ int avrt_destroy(avrt_ctx *c)
                       &c->sd[0], &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn,
                       &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
                       &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
-                      &c->rb.scratch, &c->rb.info, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
+                      &c->rb.scratch, &c->rb.info, &c->rb.depth, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
                       &c->srl[1], &c->nee_tp, &c->nee_contrib};
     for (DevBuf *b : bufs) release(*b);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
@@ -1034,7 +1064,7 @@ int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_co
                                 &c->has_tri_or_mesh, &why);
     if (rc) return set_err(c, rc, why);
     if (c->has_volume) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
-    std::vector<Node2> t2;
+    std::vector<Node4> t2;
     if (object_count) {
         uint32_t super = 0;
         if (!convert_tree(tlas, 0, node_count, t2, &super, &why)) return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + why);
@@ -1043,7 +1073,7 @@ int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_co
             if (tlas[i].count > 0 && static_cast<uint64_t>(tlas[i].left) + tlas[i].count > object_count)
                 return set_err(c, AVRT_E_INVALID_ARG, "TLAS leaf addresses slots beyond object_count");
     }
-    rc = upload_vec(c, c->d_tlas2, t2.data(), sizeof(Node2) * t2.size());
+    rc = upload_vec(c, c->d_tlas2, t2.data(), sizeof(Node4) * t2.size());
     if (rc) return rc;
     rc = upload_vec(c, c->d_geom, geom.data(), sizeof(ObjGeom) * geom.size());
     if (rc) return rc;
@@ -1232,11 +1262,24 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
 static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n, const float4 *ro, const float4 *rd,
                         const float4 *rlp)
 {
-    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + 8;
-    HIPX(c, hipMemsetAsync(work, 0, 4, c->stream));
     uint4 *hA = static_cast<uint4 *>(c->hitA.p);
     float *hB = static_cast<float *>(c->hitB.p);
-    const dim3 grid(trace_grid(c, n)), block(kBlock);
+    const dim3 block(kBlock);
+    if (c->opts.trace_schedule == 0) {
+        const dim3 grid(blocks_for(n));
+        if (c->ref_order) {
+            if (shadow) hipLaunchKernelGGL((k_trace_lane<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+            else hipLaunchKernelGGL((k_trace_lane<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+        } else {
+            if (shadow) hipLaunchKernelGGL((k_trace_lane<false, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+            else hipLaunchKernelGGL((k_trace_lane<false, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+        }
+        HIPX(c, hipGetLastError());
+        return AVRT_OK;
+    }
+    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + 8;
+    HIPX(c, hipMemsetAsync(work, 0, 4, c->stream));
+    const dim3 grid(trace_grid(c, n));
     if (c->ref_order) {
         if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
         else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);

@@ -18,7 +18,7 @@
 //                 numbering is converted to DFS preorder at the end from the
 //                 subtree sizes, so the arrays equal avrt_build_bvh's.
 //   k_pack        objects, instances and light slots in slot order
-//   k_convert     TLAS into the renderer's Node2 layout and ObjGeom records
+//   k_convert     TLAS into the renderer's Node4 layout and ObjGeom records
 //
 // Where the reference folds boxes with std::min/std::max, equal values keep
 // the first operand. The node-box reductions reproduce that (the bits of the
@@ -256,7 +256,8 @@ __device__ __forceinline__ int sah_bin(float c, float cmin, float scale)
 }
 
 __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *bounds, BuildWork w,
-                                                    AvrtBVHNode *out, uint32_t *order, uint32_t *info)
+                                                    AvrtBVHNode *out, uint32_t *out_depth, uint32_t *order,
+                                                    uint32_t *info)
 {
     __shared__ uint32_t sh_wave[kBT / 64];
     __shared__ uint32_t sh_carry;
@@ -568,6 +569,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         n.padding = 0;
         n.pad_trailing = 0;
         out[w.t_pre[t]] = n;
+        out_depth[w.t_pre[t]] = w.t_depth[t];
     }
     for (uint32_t i = tid; i < N; i += kBT) order[i] = w.idx[i];
     if (tid == 0) info[0] = ntasks;
@@ -679,14 +681,14 @@ __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order,
     }
 }
 
-__device__ __forceinline__ void set_child(Node2 &n, int c, const AvrtBVHNode &b, uint32_t ref)
+__device__ __forceinline__ void set_child(Node4 &n, int c, const AvrtBVHNode &b, uint32_t ref)
 {
-    n.bx[c] = b.min_x;
-    n.bx[c + 2] = b.max_x;
-    n.by[c] = b.min_y;
-    n.by[c + 2] = b.max_y;
-    n.bz[c] = b.min_z;
-    n.bz[c + 2] = b.max_z;
+    n.lox[c] = b.min_x;
+    n.hix[c] = b.max_x;
+    n.loy[c] = b.min_y;
+    n.hiy[c] = b.max_y;
+    n.loz[c] = b.min_z;
+    n.hiz[c] = b.max_z;
     n.ref[c] = ref;
 }
 
@@ -703,23 +705,36 @@ __device__ __forceinline__ uint32_t ref_of(const AvrtBVHNode *nodes, uint32_t j,
     return 1u + j;
 }
 
-// Node2 index of reference node j is 1 + j; index 0 is the super-root.
-__global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, Node2 *out, uint32_t *info)
+// Node4 index of an even-depth interior node j is 1 + j (odd-depth slots stay
+// unused); index 0 is the super-root. Same expansion rule as the host
+// converter: each interior child contributes its two children.
+__global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, const uint32_t *depth, Node4 *out,
+                               uint32_t *info)
 {
     const uint32_t nn = info_in[0];
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > nn || nn == 0) return;
-    Node2 n;
+    Node4 n;
     const AvrtBVHNode far_pt{1e30f, 1e30f, 1e30f, 0.0f, 1e30f, 1e30f, 1e30f, 0, 0, 0, 0, 0};
-    for (int c = 0; c < 2; c++) set_child(n, c, far_pt, kRefInvalid);
-    n.pad[0] = n.pad[1] = 0u;
+    for (int c = 0; c < 4; c++) set_child(n, c, far_pt, kRefInvalid);
+    for (int c = 0; c < 4; c++) n.pad[c] = 0u;
     if (j == 0) {
         set_child(n, 0, nodes[0], ref_of(nodes, 0, info));
     } else {
-        const AvrtBVHNode &me = nodes[j - 1];
-        if (me.count == 0) {
-            set_child(n, 0, nodes[me.left], ref_of(nodes, static_cast<uint32_t>(me.left), info));
-            set_child(n, 1, nodes[me.right], ref_of(nodes, static_cast<uint32_t>(me.right), info));
+        const uint32_t me = j - 1;
+        const AvrtBVHNode &b = nodes[me];
+        if (b.count == 0 && (depth[me] & 1u) == 0u) {
+            int c = 0;
+            for (int side = 0; side < 2; side++) {
+                const uint32_t k = static_cast<uint32_t>(side == 0 ? b.left : b.right);
+                const AvrtBVHNode &ch = nodes[k];
+                if (ch.count == 0) {
+                    set_child(n, c++, nodes[ch.left], ref_of(nodes, static_cast<uint32_t>(ch.left), info));
+                    set_child(n, c++, nodes[ch.right], ref_of(nodes, static_cast<uint32_t>(ch.right), info));
+                } else {
+                    set_child(n, c++, ch, ref_of(nodes, k, info));
+                }
+            }
         }
     }
     out[j] = n;
@@ -800,7 +815,7 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
     size_t sbytes = 0;
     BuildWork w = carve(rb.scratch.p, n, &sbytes);
     hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kBT), 0, c->stream, n, rb.bounds.as<AvrtAABB>(), w,
-                       rb.nodes.as<AvrtBVHNode>(), rb.order.as<uint32_t>(), rb.info.as<uint32_t>());
+                       rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), rb.order.as<uint32_t>(), rb.info.as<uint32_t>());
     HIPX(c, hipGetLastError());
     if (n) {
         hipLaunchKernelGGL(k_pack, dim3(1), dim3(kBT), 0, c->stream, n, rb.order.as<uint32_t>(),
@@ -809,7 +824,8 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
                            c->d_instances.as<AvrtMeshInstance>(), c->d_geom.as<ObjGeom>(), c->d_lights.as<uint32_t>());
         HIPX(c, hipGetLastError());
         hipLaunchKernelGGL(k_convert_tlas, dim3(blocks(2 * n + 1, 256)), dim3(256), 0, c->stream, rb.info.as<uint32_t>(),
-                           rb.nodes.as<AvrtBVHNode>(), c->d_tlas2.as<Node2>(), rb.info.as<uint32_t>());
+                           rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), c->d_tlas2.as<Node4>(),
+                           rb.info.as<uint32_t>());
         HIPX(c, hipGetLastError());
     }
     HIPX(c, hipEventRecord(c->ev_b, c->stream));
@@ -818,7 +834,7 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
     HIPX(c, hipStreamSynchronize(c->stream));
     (void)hipEventElapsedTime(&rb.last_ms, c->ev_a, c->ev_b);
     if (info[1] & kErrBox) return set_err(c, AVRT_E_UNSUPPORTED, "device rebuild does not handle box objects");
-    if (info[1] & kErrLeaf) return set_err(c, AVRT_E_UNSUPPORTED, "device-built TLAS leaf too large for Node2");
+    if (info[1] & kErrLeaf) return set_err(c, AVRT_E_UNSUPPORTED, "device-built TLAS leaf too large for Node4");
     if (info[1] & kErrBuild) return set_err(c, AVRT_E_DEVICE, "device TLAS build produced an inconsistent partition");
     c->nobj = n;
     c->ntlas2 = n ? info[0] + 1 : 0;
@@ -893,7 +909,8 @@ int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *objects_in, uint32_t
     if ((rc = ensure(c, c->d_instances, sizeof(AvrtMeshInstance) * (m ? m : 1)))) return rc;
     if ((rc = ensure(c, c->d_geom, sizeof(ObjGeom) * (n ? n : 1)))) return rc;
     if ((rc = ensure(c, c->d_lights, sizeof(uint32_t) * (nl ? nl : 1)))) return rc;
-    if ((rc = ensure(c, c->d_tlas2, sizeof(Node2) * (2 * n + 1)))) return rc;
+    if ((rc = ensure(c, c->d_tlas2, sizeof(Node4) * (2 * n + 1)))) return rc;
+    if ((rc = ensure(c, rb.depth, sizeof(uint32_t) * 2 * (n ? n : 1)))) return rc;
     rb.n = object_count;
     rb.m = instance_count;
     rb.nlights = nl;

@@ -26,27 +26,44 @@ struct Converter {
     const AvrtBVHNode *nodes;
     uint32_t base;      // index of the tree's root in `nodes`
     uint32_t avail;     // nodes available from base
-    std::vector<Node2> *out;
+    std::vector<Node4> *out;
     std::string err;
 
-    static void set_child(Node2 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
+    static void set_child(Node4 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
     {
-        n.bx[c] = lo[0];
-        n.bx[c + 2] = hi[0];
-        n.by[c] = lo[1];
-        n.by[c + 2] = hi[1];
-        n.bz[c] = lo[2];
-        n.bz[c + 2] = hi[2];
+        n.lox[c] = lo[0];
+        n.hix[c] = hi[0];
+        n.loy[c] = lo[1];
+        n.hiy[c] = hi[1];
+        n.loz[c] = lo[2];
+        n.hiz[c] = hi[2];
         n.ref[c] = ref;
     }
-    static void set_empty(Node2 &n, int c)
+    static Node4 empty_node()
     {
+        Node4 n{};
         const float far_pt[3] = {1e30f, 1e30f, 1e30f};
-        set_child(n, c, far_pt, far_pt, kRefInvalid);
+        for (int c = 0; c < 4; c++) set_child(n, c, far_pt, far_pt, kRefInvalid);
+        return n;
+    }
+    bool box_of(uint32_t local, float lo[3], float hi[3])
+    {
+        if (local >= avail) {
+            err = "BVH child index out of range";
+            return false;
+        }
+        const AvrtBVHNode &n = nodes[base + local];
+        lo[0] = n.min_x;
+        lo[1] = n.min_y;
+        lo[2] = n.min_z;
+        hi[0] = n.max_x;
+        hi[1] = n.max_y;
+        hi[2] = n.max_z;
+        return true;
     }
 
     // Reference for a leaf [first, first+count) whose box is lo/hi. Leaves
-    // over 127 primitives become small subtrees of the same box.
+    // over 127 primitives become small subtrees over the same box.
     bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref)
     {
         if (count == 0) {
@@ -62,7 +79,7 @@ struct Converter {
             return true;
         }
         const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
+        out->push_back(empty_node());
         const uint32_t half = count / 2;
         uint32_t r0, r1;
         if (!leaf_ref(first, half, lo, hi, &r0)) return false;
@@ -73,6 +90,8 @@ struct Converter {
         return true;
     }
 
+    // Node4 of binary interior node `local`: its children, each interior
+    // child replaced by its own two children, left to right.
     bool ref_of(uint32_t local, int depth, uint32_t *ref)
     {
         if (local >= avail || depth > 64) {
@@ -81,30 +100,41 @@ struct Converter {
         }
         const AvrtBVHNode &n = nodes[base + local];
         if (n.count > 0) {
-            const float lo[3] = {n.min_x, n.min_y, n.min_z};
-            const float hi[3] = {n.max_x, n.max_y, n.max_z};
+            float lo[3], hi[3];
+            box_of(local, lo, hi);
             if (n.left < 0) {
                 err = "negative leaf slot";
                 return false;
             }
             return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref);
         }
-        if (n.left < 0 || n.right < 0) {
-            err = "negative child index";
+        if (n.left < 0 || n.right < 0 || static_cast<uint32_t>(n.left) >= avail || static_cast<uint32_t>(n.right) >= avail) {
+            err = "bad child index";
             return false;
         }
-        const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
-        uint32_t r[2];
-        const uint32_t kids[2] = {static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.right)};
-        for (int c = 0; c < 2; c++) {
-            if (!ref_of(kids[c], depth + 1, &r[c])) return false;
+        uint32_t kids[4];
+        int nk = 0;
+        for (int32_t c : {n.left, n.right}) {
+            const AvrtBVHNode &ch = nodes[base + c];
+            if (ch.count == 0) {
+                if (ch.left < 0 || ch.right < 0) {
+                    err = "bad child index";
+                    return false;
+                }
+                kids[nk++] = static_cast<uint32_t>(ch.left);
+                kids[nk++] = static_cast<uint32_t>(ch.right);
+            } else {
+                kids[nk++] = static_cast<uint32_t>(c);
+            }
         }
-        for (int c = 0; c < 2; c++) {
-            const AvrtBVHNode &ch = nodes[base + kids[c]];
-            const float lo[3] = {ch.min_x, ch.min_y, ch.min_z};
-            const float hi[3] = {ch.max_x, ch.max_y, ch.max_z};
-            set_child((*out)[me], c, lo, hi, r[c]);
+        const uint32_t me = static_cast<uint32_t>(out->size());
+        out->push_back(empty_node());
+        for (int c = 0; c < nk; c++) {
+            uint32_t r;
+            if (!ref_of(kids[c], depth + 2, &r)) return false;
+            float lo[3], hi[3];
+            if (!box_of(kids[c], lo, hi)) return false;
+            set_child((*out)[me], c, lo, hi, r);
         }
         *ref = me;
         return true;
@@ -115,14 +145,12 @@ struct Converter {
     bool convert(uint32_t *super)
     {
         const uint32_t me = static_cast<uint32_t>(out->size());
-        out->push_back(Node2{});
+        out->push_back(empty_node());
         uint32_t r;
         if (!ref_of(0, 0, &r)) return false;
-        const AvrtBVHNode &root = nodes[base];
-        const float lo[3] = {root.min_x, root.min_y, root.min_z};
-        const float hi[3] = {root.max_x, root.max_y, root.max_z};
+        float lo[3], hi[3];
+        box_of(0, lo, hi);
         set_child((*out)[me], 0, lo, hi, r);
-        set_empty((*out)[me], 1);
         *super = me;
         return true;
     }
@@ -130,7 +158,7 @@ struct Converter {
 
 } // namespace
 
-bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node2> &out, uint32_t *super,
+bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
                   std::string *err)
 {
     Converter cv{nodes, base, avail, &out, {}};

@@ -16,7 +16,7 @@ namespace avrt {
 // MeshLibrary::Add stores a BLAS and BuildBVH a TLAS) behind a super-root to
 // `out`; *super receives the super-root's index. Leaves of more than 127
 // primitives become small subtrees over the same box.
-bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node2> &out, uint32_t *super,
+bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
                   std::string *err);
 
 // Triangle record of HitMeshTriangle's derived quantities.
