@@ -38,6 +38,8 @@ struct SceneRebuild {
     DevBuf bounds;           // AvrtAABB[n]
     DevBuf nodes;            // AvrtBVHNode[2n]
     DevBuf depth;            // uint32[2n]: depth of each node
+    DevBuf range;            // uint2[2n]: slot range [start, end) of each node
+    DevBuf lg_prefix;        // uint32[n+1]: light/glass objects in slots [0, s)
     DevBuf order;            // uint32[n]
     DevBuf scratch;          // builder work arrays
     DevBuf info;             // uint32[8]: node count, error flags

@@ -303,12 +303,24 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
 // then in `hit`. REF_ORDER=true visits children like the reference (right
 // child first), so hits at exactly equal t resolve as there; false visits the
 // nearer child first.
-template <bool REF_ORDER, int K, int BLOCK, int SPILL>
+//
+// FILTER restricts the objects considered:
+//   kAll        every object, closest hit;
+//   kLightGlass only lights and glass (kGeomLG), closest hit;
+//   kOpaqueAny  only the other objects, and the walk ends at the first hit
+//               closer than `term`: the any-hit occlusion test that skips
+//               glass and lights (Occluded, deferred.comp.hlsl:1115-1171).
+// Child masks in Node4::pad[0] prune subtrees that hold no object of the
+// requested class.
+constexpr int kAll = 0, kLightGlass = 1, kOpaqueAny = 2;
+
+template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll>
 struct Traversal {
     F3 o, d, winv;    // world ray
     F3 ao, ainv;      // active ray (world, or mesh-local inside a BLAS)
     F3 ad;
     float time, near, far;
+    float term; // kOpaqueAny: a hit closer than this ends the walk
     uint32_t ref;
     uint32_t cur_obj, tri_base;
     bool in_blas;
@@ -329,6 +341,7 @@ struct Traversal {
         hit.t = far_;
         hit.alpha = 0.0f;
         hit.beta = 0.0f;
+        term = far_;
         winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         ao = o;
         ad = d;
@@ -372,10 +385,17 @@ struct Traversal {
             const float4 HZ = *reinterpret_cast<const float4 *>(np->hiz);
             const uint4 R = *reinterpret_cast<const uint4 *>(np->ref);
             float t0, t1, t2, t3;
-            const bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, ao, ainv, near, far, &t0) && R.x != kRefInvalid;
-            const bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, ao, ainv, near, far, &t1) && R.y != kRefInvalid;
-            const bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, ao, ainv, near, far, &t2) && R.z != kRefInvalid;
-            const bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, ao, ainv, near, far, &t3) && R.w != kRefInvalid;
+            bool h0 = slab(LX.x, HX.x, LY.x, HY.x, LZ.x, HZ.x, ao, ainv, near, far, &t0) && R.x != kRefInvalid;
+            bool h1 = slab(LX.y, HX.y, LY.y, HY.y, LZ.y, HZ.y, ao, ainv, near, far, &t1) && R.y != kRefInvalid;
+            bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, ao, ainv, near, far, &t2) && R.z != kRefInvalid;
+            bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, ao, ainv, near, far, &t3) && R.w != kRefInvalid;
+            if (FILTER != kAll) {
+                const uint32_t m = np->pad[0] >> (FILTER == kLightGlass ? 0 : 4);
+                h0 = h0 && (m & 1u);
+                h1 = h1 && (m & 2u);
+                h2 = h2 && (m & 4u);
+                h3 = h3 && (m & 8u);
+            }
             if (REF_ORDER) {
                 // right to left: the highest-index child next, the others
                 // pushed left to right so they pop right to left
@@ -450,6 +470,7 @@ struct Traversal {
                 hit.prim = first + i;
                 hit.alpha = al;
                 hit.beta = be;
+                if (FILTER == kOpaqueAny && t < term) return false;
             }
             return next();
         }
@@ -463,6 +484,8 @@ struct Traversal {
             *reinterpret_cast<uint4 *>(g.d) = *reinterpret_cast<const uint4 *>(gp->d);
             const uint32_t flags = g.d[3];
             if ((flags & kGeomValid) == 0) continue;
+            if (FILTER == kLightGlass && (flags & kGeomLG) == 0) continue;
+            if (FILTER == kOpaqueAny && (flags & kGeomLG) != 0) continue;
             const uint32_t shape = flags & 0xFFu;
             float t = 0.0f, al = 0.0f, be = 0.0f;
             bool h = false;
@@ -501,6 +524,7 @@ struct Traversal {
                 hit.prim = ~0u;
                 hit.alpha = al;
                 hit.beta = be;
+                if (FILTER == kOpaqueAny && t < term) return false;
             }
         }
         return next();

@@ -15,6 +15,10 @@
 //                 interior = Node4 index; leaf = LEAF | count << 24 | first.
 //                 Each tree starts with a "super-root" whose child 0 is the
 //                 reference root (so the root box is tested, bvh.h:275).
+//                 pad[0] classifies the children's subtrees for shadow
+//                 queries: bit c = child c holds a light or glass object,
+//                 bit 4 + c = child c holds any other object (BLAS nodes:
+//                 all bits set).
 //   TriPre  64 B  per mesh triangle in BLAS slot order: p0, e1, e2 and
 //                 w = n / |n|^2 plus dot(n, p0), exactly the quantities
 //                 HitMeshTriangle (ray_trace.comp.hybrid.hlsl:564-598)
@@ -64,8 +68,17 @@ struct alignas(16) TriPre {
 };
 static_assert(sizeof(TriPre) == 64, "TriPre is 64 bytes");
 
-// ObjGeom word 15 (flags): bits 0-7 shape, bit 8 valid.
+// ObjGeom word 15 (flags): bits 0-7 shape, bit 8 valid, bit 9 the object is
+// a light (DIFFUSE_LIGHT) or glass (DIELECTRIC): the two materials a shadow
+// ray does not simply stop at.
 constexpr uint32_t kGeomValid = 1u << 8;
+constexpr uint32_t kGeomLG = 1u << 9;
+// Subtree classes of the Node4 child mask.
+constexpr uint32_t kClassLG = 1u, kClassOther = 2u;
+inline constexpr uint32_t child_mask_bits(int c, uint32_t cls)
+{
+    return ((cls & kClassLG) ? (1u << c) : 0u) | ((cls & kClassOther) ? (1u << (4 + c)) : 0u);
+}
 
 struct alignas(16) ObjGeom {
     float a[4];

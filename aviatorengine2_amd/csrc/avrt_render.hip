@@ -298,7 +298,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_lane(SceneDev sc, uint32_t n, 
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= n) return;
     uint32_t spill_mem[kStackSpill];
-    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
     const float4 O = ro[idx];
     const float4 D = rd[idx];
     const F3 o = f3(O.x, O.y, O.z);
@@ -307,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_lane(SceneDev sc, uint32_t n, 
         const float4 L = rlp[idx];
         far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
     }
+    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
     if (tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem)) {
         while (tr.step(sc)) {
         }
@@ -995,7 +995,7 @@ int avrt_destroy(avrt_ctx *c)
                       &c->sd[0], &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn,
                       &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
                       &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
-                      &c->rb.scratch, &c->rb.info, &c->rb.depth, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
+                      &c->rb.scratch, &c->rb.info, &c->rb.depth, &c->rb.range, &c->rb.lg_prefix, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
                       &c->srl[1], &c->nee_tp, &c->nee_contrib};
     for (DevBuf *b : bufs) release(*b);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
@@ -1067,7 +1067,10 @@ int avrt_upload_scene(avrt_ctx *c, const AvrtObject *objects, uint32_t object_co
     std::vector<Node4> t2;
     if (object_count) {
         uint32_t super = 0;
-        if (!convert_tree(tlas, 0, node_count, t2, &super, &why)) return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + why);
+        std::vector<uint8_t> cls(object_count);
+        for (uint32_t i = 0; i < object_count; i++) cls[i] = object_class(objects[i]);
+        if (!convert_tree(tlas, 0, node_count, t2, &super, &why, cls.data(), object_count))
+            return set_err(c, AVRT_E_INVALID_ARG, "TLAS: " + why);
         // leaf slots must address uploaded objects
         for (uint32_t i = 0; i < node_count; i++)
             if (tlas[i].count > 0 && static_cast<uint64_t>(tlas[i].left) + tlas[i].count > object_count)
@@ -1265,7 +1268,7 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
     uint4 *hA = static_cast<uint4 *>(c->hitA.p);
     float *hB = static_cast<float *>(c->hitB.p);
     const dim3 block(kBlock);
-    if (c->opts.trace_schedule == 0) {
+    if (c->opts.trace_schedule == 1) {
         const dim3 grid(blocks_for(n));
         if (c->ref_order) {
             if (shadow) hipLaunchKernelGGL((k_trace_lane<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);

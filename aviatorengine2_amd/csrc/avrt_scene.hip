@@ -256,8 +256,8 @@ __device__ __forceinline__ int sah_bin(float c, float cmin, float scale)
 }
 
 __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *bounds, BuildWork w,
-                                                    AvrtBVHNode *out, uint32_t *out_depth, uint32_t *order,
-                                                    uint32_t *info)
+                                                    AvrtBVHNode *out, uint32_t *out_depth, uint2 *out_range,
+                                                    uint32_t *order, uint32_t *info)
 {
     __shared__ uint32_t sh_wave[kBT / 64];
     __shared__ uint32_t sh_carry;
@@ -570,6 +570,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         n.pad_trailing = 0;
         out[w.t_pre[t]] = n;
         out_depth[w.t_pre[t]] = w.t_depth[t];
+        out_range[w.t_pre[t]] = make_uint2(w.t_start[t], w.t_end[t]);
     }
     for (uint32_t i = tid; i < N; i += kBT) order[i] = w.idx[i];
     if (tid == 0) info[0] = ntasks;
@@ -612,21 +613,25 @@ __device__ ObjGeom make_geom(const AvrtObject &o, const AvrtMeshInstance *inst, 
         g.c[2] = __uint_as_float(c.z); g.c[3] = __uint_as_float(c.w);
         if (m.triangleCount > 0) flags |= kGeomValid;
     }
+    if (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC) flags |= kGeomLG;
     g.d[3] = flags;
     return g;
 }
 
+// Also writes lg_prefix[s] = number of light or glass objects in slots
+// [0, s), for s = 0..n (the child masks of k_convert_tlas).
 __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order, const AvrtObject *auth,
                                               const int32_t *mesh_rank, const AvrtMeshInstance *auth_inst,
                                               const uint4 *geomc, AvrtObject *objects, AvrtMeshInstance *instances,
-                                              ObjGeom *geom, uint32_t *lights)
+                                              ObjGeom *geom, uint32_t *lights, uint32_t *lg_prefix)
 {
-    __shared__ uint32_t sh_wm[kBT / 64], sh_wl[kBT / 64];
-    __shared__ uint32_t sh_cm, sh_cl;
+    __shared__ uint32_t sh_wm[kBT / 64], sh_wl[kBT / 64], sh_wg[kBT / 64];
+    __shared__ uint32_t sh_cm, sh_cl, sh_cg;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) {
         sh_cm = 0;
         sh_cl = 0;
+        sh_cg = 0;
     }
     __syncthreads();
     for (uint32_t base = 0; base < n; base += kBT) {
@@ -636,32 +641,38 @@ __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order,
         AvrtObject o = auth[a];
         const uint32_t im = (valid && o.shapeType == AVRT_SHAPE_MESH) ? 1u : 0u;
         const uint32_t il = (valid && o.colorType == AVRT_DIFFUSE_LIGHT) ? 1u : 0u;
-        uint32_t xm = im, xl = il;
+        const uint32_t ig = (valid && (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC)) ? 1u : 0u;
+        uint32_t xm = im, xl = il, xg = ig;
         for (uint32_t off = 1; off < 64; off <<= 1) {
-            const uint32_t ym = __shfl_up(xm, off), yl = __shfl_up(xl, off);
+            const uint32_t ym = __shfl_up(xm, off), yl = __shfl_up(xl, off), yg = __shfl_up(xg, off);
             if (lane >= off) {
                 xm += ym;
                 xl += yl;
+                xg += yg;
             }
         }
         if (lane == 63u) {
             sh_wm[wave] = xm;
             sh_wl[wave] = xl;
+            sh_wg[wave] = xg;
         }
         __syncthreads();
         if (tid == 0) {
-            uint32_t am = 0, al = 0;
+            uint32_t am = 0, al = 0, ag = 0;
             for (uint32_t k = 0; k < kBT / 64; k++) {
-                const uint32_t vm = sh_wm[k], vl = sh_wl[k];
+                const uint32_t vm = sh_wm[k], vl = sh_wl[k], vg = sh_wg[k];
                 sh_wm[k] = am;
                 sh_wl[k] = al;
+                sh_wg[k] = ag;
                 am += vm;
                 al += vl;
+                ag += vg;
             }
         }
         __syncthreads();
         const uint32_t rm = sh_cm + sh_wm[wave] + xm - im;
         const uint32_t rl = sh_cl + sh_wl[wave] + xl - il;
+        const uint32_t rg = sh_cg + sh_wg[wave] + xg - ig;
         if (valid) {
             const int32_t rank = mesh_rank[a];
             if (im) {
@@ -671,11 +682,14 @@ __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order,
             objects[s] = o;
             geom[s] = make_geom(o, auth_inst, geomc, rank);
             if (il) lights[rl] = s;
+            lg_prefix[s] = rg;
+            if (s == n - 1) lg_prefix[n] = rg + ig;
         }
         __syncthreads();
         if (tid == kBT - 1) {
             sh_cm = rm + im;
             sh_cl = rl + il;
+            sh_cg = rg + ig;
         }
         __syncthreads();
     }
@@ -705,11 +719,19 @@ __device__ __forceinline__ uint32_t ref_of(const AvrtBVHNode *nodes, uint32_t j,
     return 1u + j;
 }
 
+__device__ __forceinline__ uint32_t class_of(const uint2 *range, const uint32_t *lg_prefix, uint32_t j)
+{
+    const uint2 r = range[j];
+    const uint32_t lg = lg_prefix[r.y] - lg_prefix[r.x];
+    return (lg ? kClassLG : 0u) | ((r.y - r.x) > lg ? kClassOther : 0u);
+}
+
 // Node4 index of an even-depth interior node j is 1 + j (odd-depth slots stay
 // unused); index 0 is the super-root. Same expansion rule as the host
-// converter: each interior child contributes its two children.
-__global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, const uint32_t *depth, Node4 *out,
-                               uint32_t *info)
+// converter: each interior child contributes its two children. Child masks
+// come from the slot range of each child's subtree.
+__global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, const uint32_t *depth,
+                               const uint2 *range, const uint32_t *lg_prefix, Node4 *out, uint32_t *info)
 {
     const uint32_t nn = info_in[0];
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -718,8 +740,10 @@ __global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes
     const AvrtBVHNode far_pt{1e30f, 1e30f, 1e30f, 0.0f, 1e30f, 1e30f, 1e30f, 0, 0, 0, 0, 0};
     for (int c = 0; c < 4; c++) set_child(n, c, far_pt, kRefInvalid);
     for (int c = 0; c < 4; c++) n.pad[c] = 0u;
+    uint32_t mask = 0;
     if (j == 0) {
         set_child(n, 0, nodes[0], ref_of(nodes, 0, info));
+        mask = child_mask_bits(0, class_of(range, lg_prefix, 0));
     } else {
         const uint32_t me = j - 1;
         const AvrtBVHNode &b = nodes[me];
@@ -729,14 +753,21 @@ __global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes
                 const uint32_t k = static_cast<uint32_t>(side == 0 ? b.left : b.right);
                 const AvrtBVHNode &ch = nodes[k];
                 if (ch.count == 0) {
-                    set_child(n, c++, nodes[ch.left], ref_of(nodes, static_cast<uint32_t>(ch.left), info));
-                    set_child(n, c++, nodes[ch.right], ref_of(nodes, static_cast<uint32_t>(ch.right), info));
+                    for (int g = 0; g < 2; g++) {
+                        const uint32_t q = static_cast<uint32_t>(g == 0 ? ch.left : ch.right);
+                        set_child(n, c, nodes[q], ref_of(nodes, q, info));
+                        mask |= child_mask_bits(c, class_of(range, lg_prefix, q));
+                        c++;
+                    }
                 } else {
-                    set_child(n, c++, ch, ref_of(nodes, k, info));
+                    set_child(n, c, ch, ref_of(nodes, k, info));
+                    mask |= child_mask_bits(c, class_of(range, lg_prefix, k));
+                    c++;
                 }
             }
         }
     }
+    n.pad[0] = mask;
     out[j] = n;
 }
 
@@ -815,17 +846,19 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
     size_t sbytes = 0;
     BuildWork w = carve(rb.scratch.p, n, &sbytes);
     hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kBT), 0, c->stream, n, rb.bounds.as<AvrtAABB>(), w,
-                       rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), rb.order.as<uint32_t>(), rb.info.as<uint32_t>());
+                       rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), rb.range.as<uint2>(), rb.order.as<uint32_t>(),
+                       rb.info.as<uint32_t>());
     HIPX(c, hipGetLastError());
     if (n) {
         hipLaunchKernelGGL(k_pack, dim3(1), dim3(kBT), 0, c->stream, n, rb.order.as<uint32_t>(),
                            rb.auth_objects.as<AvrtObject>(), rb.mesh_rank.as<int32_t>(),
                            rb.auth_inst.as<AvrtMeshInstance>(), rb.inst_geomc.as<uint4>(), c->d_objects.as<AvrtObject>(),
-                           c->d_instances.as<AvrtMeshInstance>(), c->d_geom.as<ObjGeom>(), c->d_lights.as<uint32_t>());
+                           c->d_instances.as<AvrtMeshInstance>(), c->d_geom.as<ObjGeom>(), c->d_lights.as<uint32_t>(),
+                           rb.lg_prefix.as<uint32_t>());
         HIPX(c, hipGetLastError());
         hipLaunchKernelGGL(k_convert_tlas, dim3(blocks(2 * n + 1, 256)), dim3(256), 0, c->stream, rb.info.as<uint32_t>(),
-                           rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), c->d_tlas2.as<Node4>(),
-                           rb.info.as<uint32_t>());
+                           rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), rb.range.as<uint2>(),
+                           rb.lg_prefix.as<uint32_t>(), c->d_tlas2.as<Node4>(), rb.info.as<uint32_t>());
         HIPX(c, hipGetLastError());
     }
     HIPX(c, hipEventRecord(c->ev_b, c->stream));
@@ -911,6 +944,8 @@ int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *objects_in, uint32_t
     if ((rc = ensure(c, c->d_lights, sizeof(uint32_t) * (nl ? nl : 1)))) return rc;
     if ((rc = ensure(c, c->d_tlas2, sizeof(Node4) * (2 * n + 1)))) return rc;
     if ((rc = ensure(c, rb.depth, sizeof(uint32_t) * 2 * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, rb.range, sizeof(uint2) * 2 * (n ? n : 1)))) return rc;
+    if ((rc = ensure(c, rb.lg_prefix, sizeof(uint32_t) * (n + 1)))) return rc;
     rb.n = object_count;
     rb.m = instance_count;
     rb.nlights = nl;

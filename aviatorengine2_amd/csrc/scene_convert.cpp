@@ -1,5 +1,5 @@
 // scene_convert.cpp - host conversion of the reference ABI arrays into the
-// traversal layouts (Node2 trees, TriPre records, ObjGeom records).
+// traversal layouts (Node4 trees, TriPre records, ObjGeom records).
 //
 // Every derived float is computed with the same operations, in the same
 // order, as the reference computes it per ray (HitQuad / HitTriangle /
@@ -27,6 +27,8 @@ struct Converter {
     uint32_t base;      // index of the tree's root in `nodes`
     uint32_t avail;     // nodes available from base
     std::vector<Node4> *out;
+    const uint8_t *slot_class; // null: every subtree holds both classes
+    uint32_t nslots;
     std::string err;
 
     static void set_child(Node4 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
@@ -46,6 +48,13 @@ struct Converter {
         for (int c = 0; c < 4; c++) set_child(n, c, far_pt, far_pt, kRefInvalid);
         return n;
     }
+    uint32_t class_of_range(uint32_t first, uint32_t count)
+    {
+        if (slot_class == nullptr) return kClassLG | kClassOther;
+        uint32_t cls = 0;
+        for (uint32_t i = first; i < first + count && i < nslots; i++) cls |= slot_class[i];
+        return cls;
+    }
     bool box_of(uint32_t local, float lo[3], float hi[3])
     {
         if (local >= avail) {
@@ -64,7 +73,7 @@ struct Converter {
 
     // Reference for a leaf [first, first+count) whose box is lo/hi. Leaves
     // over 127 primitives become small subtrees over the same box.
-    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref)
+    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref, uint32_t *cls)
     {
         if (count == 0) {
             err = "empty leaf";
@@ -76,23 +85,26 @@ struct Converter {
         }
         if (count <= kMaxLeafCount) {
             *ref = make_leaf(first, count);
+            *cls = class_of_range(first, count);
             return true;
         }
         const uint32_t me = static_cast<uint32_t>(out->size());
         out->push_back(empty_node());
         const uint32_t half = count / 2;
-        uint32_t r0, r1;
-        if (!leaf_ref(first, half, lo, hi, &r0)) return false;
-        if (!leaf_ref(first + half, count - half, lo, hi, &r1)) return false;
+        uint32_t r0, r1, c0, c1;
+        if (!leaf_ref(first, half, lo, hi, &r0, &c0)) return false;
+        if (!leaf_ref(first + half, count - half, lo, hi, &r1, &c1)) return false;
         set_child((*out)[me], 0, lo, hi, r0);
         set_child((*out)[me], 1, lo, hi, r1);
+        (*out)[me].pad[0] = child_mask_bits(0, c0) | child_mask_bits(1, c1);
         *ref = me;
+        *cls = c0 | c1;
         return true;
     }
 
     // Node4 of binary interior node `local`: its children, each interior
     // child replaced by its own two children, left to right.
-    bool ref_of(uint32_t local, int depth, uint32_t *ref)
+    bool ref_of(uint32_t local, int depth, uint32_t *ref, uint32_t *cls)
     {
         if (local >= avail || depth > 64) {
             err = "BVH child index out of range";
@@ -106,7 +118,7 @@ struct Converter {
                 err = "negative leaf slot";
                 return false;
             }
-            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref);
+            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref, cls);
         }
         if (n.left < 0 || n.right < 0 || static_cast<uint32_t>(n.left) >= avail || static_cast<uint32_t>(n.right) >= avail) {
             err = "bad child index";
@@ -129,14 +141,19 @@ struct Converter {
         }
         const uint32_t me = static_cast<uint32_t>(out->size());
         out->push_back(empty_node());
+        uint32_t mask = 0, all = 0;
         for (int c = 0; c < nk; c++) {
-            uint32_t r;
-            if (!ref_of(kids[c], depth + 2, &r)) return false;
+            uint32_t r, cc;
+            if (!ref_of(kids[c], depth + 2, &r, &cc)) return false;
             float lo[3], hi[3];
             if (!box_of(kids[c], lo, hi)) return false;
             set_child((*out)[me], c, lo, hi, r);
+            mask |= child_mask_bits(c, cc);
+            all |= cc;
         }
+        (*out)[me].pad[0] = mask;
         *ref = me;
+        *cls = all;
         return true;
     }
 
@@ -146,11 +163,12 @@ struct Converter {
     {
         const uint32_t me = static_cast<uint32_t>(out->size());
         out->push_back(empty_node());
-        uint32_t r;
-        if (!ref_of(0, 0, &r)) return false;
+        uint32_t r, cls;
+        if (!ref_of(0, 0, &r, &cls)) return false;
         float lo[3], hi[3];
         box_of(0, lo, hi);
         set_child((*out)[me], 0, lo, hi, r);
+        (*out)[me].pad[0] = child_mask_bits(0, cls);
         *super = me;
         return true;
     }
@@ -158,10 +176,15 @@ struct Converter {
 
 } // namespace
 
-bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
-                  std::string *err)
+uint8_t object_class(const AvrtObject &o)
 {
-    Converter cv{nodes, base, avail, &out, {}};
+    return (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC) ? kClassLG : kClassOther;
+}
+
+bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
+                  std::string *err, const uint8_t *slot_class, uint32_t nslots)
+{
+    Converter cv{nodes, base, avail, &out, slot_class, nslots, {}};
     if (!cv.convert(super)) {
         *err = cv.err;
         return false;
@@ -252,6 +275,7 @@ int build_geom_records(const AvrtObject *objects, uint32_t n, const AvrtMeshInst
             break;
         }
         }
+        if (object_class(o) == kClassLG) flags |= kGeomLG;
         g.d[3] = flags;
     }
     return AVRT_OK;

@@ -15,9 +15,14 @@ namespace avrt {
 // Appends the tree rooted at nodes[base] (children relative to base, as
 // MeshLibrary::Add stores a BLAS and BuildBVH a TLAS) behind a super-root to
 // `out`; *super receives the super-root's index. Leaves of more than 127
-// primitives become small subtrees over the same box.
+// primitives become small subtrees over the same box. `slot_class` (kClassLG
+// / kClassOther per leaf slot, `nslots` of them) fills the child masks; null
+// marks every child as holding both classes.
 bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
-                  std::string *err);
+                  std::string *err, const uint8_t *slot_class = nullptr, uint32_t nslots = 0);
+
+// kClassLG for DIFFUSE_LIGHT and DIELECTRIC objects, else kClassOther.
+uint8_t object_class(const AvrtObject &o);
 
 // Triangle record of HitMeshTriangle's derived quantities.
 TriPre make_tripre(const float p0[3], const float p1[3], const float p2[3]);

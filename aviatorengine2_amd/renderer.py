@@ -31,7 +31,7 @@ class Renderer:
 
     # -- lifecycle ---------------------------------------------------------
     def Initialize(self, device=0, max_width=0, max_height=0, max_paths=0, fp16_accum=False, reference_order=False,
-                   persistent_trace=False):
+                   lane_trace=False):
         self.lib = abi.load_library()
         opts = AvrtCreateOptions()
         opts.device = device
@@ -40,7 +40,7 @@ class Renderer:
         opts.max_paths = max_paths
         opts.fp16_accum = 1 if fp16_accum else 0
         opts.traversal_order = 1 if reference_order else 0
-        opts.trace_schedule = 1 if persistent_trace else 0
+        opts.trace_schedule = 1 if lane_trace else 0
         ctx = ctypes.c_void_p()
         rc = self.lib.avrt_create(ctypes.byref(opts), ctypes.byref(ctx))
         if rc:

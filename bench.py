@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=8,
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--persistent", action="store_true", help="persistent traversal lanes with ray refill")
+    ap.add_argument("--lane-trace", action="store_true", help="one ray per lane instead of persistent lanes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -84,7 +84,7 @@ def main():
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {len(packed.objects)} objects, "
         f"{len(packed.indices) // 3} BLAS triangles, {len(packed.blas)} BLAS nodes")
 
-    r = Renderer().Initialize(device=0, persistent_trace=args.persistent)
+    r = Renderer().Initialize(device=0, lane_trace=args.lane_trace)
     r.UploadMeshGeometry(library[0], library[1], library[2])
     r.UploadTextures(spec.texture)
     r.SetAuthoring(objs, inst, local)

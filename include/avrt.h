@@ -193,9 +193,9 @@ typedef struct AvrtCreateOptions {
                                reference's order, right child first
                                (ray_trace.comp.hybrid.hlsl:909-913), so hits
                                at exactly equal t resolve as in the reference */
-    uint32_t trace_schedule; /* 0: one ray per lane until it finishes. 1:
-                               persistent lanes that take a new ray when theirs
-                               finishes (same hits, different speed)          */
+    uint32_t trace_schedule; /* 0: persistent lanes that take a new ray when
+                               theirs finishes (fastest measured). 1: one ray
+                               per lane until it finishes. Same hits.         */
     uint32_t reserved[1];
 } AvrtCreateOptions;
 

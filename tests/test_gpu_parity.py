@@ -117,6 +117,20 @@ def test_render_sphere_field_path_trace(c1):
     assert stats["rays_primary"] == 400 * 225 * 3
 
 
+def test_render_sphere_field_ray_trace_glass_shadows():
+    """Many glass spheres lit by emissive spheres and quads: shadow rays step
+    through glass (the light/glass walk plus the opaque any-hit walk must
+    classify as the closest hit would)."""
+    pk = _with_lights(scenes.sphere_field(), 3)
+    assert (pk.objects["colorType"] == abi.DIELECTRIC).sum() > 10
+    cfg = small_cfg(pk.config, 320, 180, 2)
+    say("render sphere_field + 3 lights 320x180x2 ray_trace (glass shadows)")
+    img, _, stats = render_gpu(pk, cfg, abi.MODE_RAY_TRACE)
+    ref, _, _, _ = oracle.render(pk, cfg, abi.MODE_RAY_TRACE)
+    _compare("sphere_field ray_trace", img, ref)
+    assert stats["rays_shadow"] > 0
+
+
 def test_render_icosphere_tris_ray_trace(c2):
     cfg = small_cfg(c2.config, 320, 180, 2)
     say("render icosphere_tris 320x180x2 ray_trace")
@@ -145,8 +159,9 @@ def test_render_reference_order_is_bit_exact(c3):
 
 
 def _with_lights(spec, count):
-    """The icosphere scene with `count` lights: the quad plus extra emissive
-    spheres and quads around it."""
+    """`spec` with count - 1 extra lights, emissive spheres and quads on a
+    ring of radius 3 above the origin (the icosphere scene's quad light makes
+    `count` lights there)."""
     extra = []
     for k in range(count - 1):
         a = 2 * np.pi * k / max(1, count - 1)
@@ -208,6 +223,19 @@ def test_chunked_paths_equal_single_wave(c2):
         r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
         b = r.ReadImage()
     assert np.array_equal(a, b)
+
+
+def test_trace_schedules_agree(c3):
+    """Persistent lanes (default) and one ray per lane give the same image."""
+    say("persistent vs lane traversal schedule")
+    cfg = small_cfg(c3.config, 160, 90, 2)
+    out = []
+    for lane in (False, True):
+        with Renderer().Initialize(lane_trace=lane) as r:
+            r.LoadScene(c3)
+            r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
+            out.append(r.ReadImage())
+    assert np.array_equal(out[0], out[1])
 
 
 def test_empty_scene_renders_sky():
