@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/avrt.h"
+#include "avrt_device.h"
 #include "avrt_internal.h"
 
 namespace avrt {
@@ -40,11 +41,32 @@ struct SceneRebuild {
     DevBuf depth;            // uint32[2n]: depth of each node
     DevBuf range;            // uint2[2n]: slot range [start, end) of each node
     DevBuf lg_prefix;        // uint32[n+1]: light/glass objects in slots [0, s)
+    DevBuf raster_prefix;    // uint32[n+1]: rasterized objects in slots [0, s)
     DevBuf order;            // uint32[n]
     DevBuf scratch;          // builder work arrays
     DevBuf info;             // uint32[8]: node count, error flags
     size_t scratch_bytes = 0;
     float last_ms = 0.0f;
+};
+
+// Targets and queues of the hybrid frame (avrt_render_hybrid): the ray-cast
+// G-buffer at full resolution, the two history slots (accumulation image and
+// per-pixel geometry), the reduced-grid lighting and the per-cell state of
+// the deferred pass.
+struct HybridState {
+    uint32_t w = 0, h = 0;       // full resolution the targets are sized for
+    size_t cap_cells = 0;        // per-cell queues sized for this many cells
+    size_t cap_shadow = 0;
+    int hist = 0;                // history slot written last
+    bool any = false;            // a hybrid frame has run
+    DevBuf gpos, gnrm, galb, gdep;       // float4, half4, uchar4, float
+    DevBuf img[2], geom[2];              // float4 each
+    DevBuf lred;                         // float4 per cell
+    DevBuf pr_o, pr_d, pr_far, pr_hit, pr_hb, rng, lcol, lind;
+    DevBuf po[2], pd[2], pt[2];          // bounce path queues
+    DevBuf hitA, hitB;
+    DevBuf so, sd, sfar, nee_tp, nee_contrib;
+    DevBuf counters;
 };
 
 } // namespace avrt
@@ -91,6 +113,8 @@ struct avrt_ctx {
     int trace_blocks_per_cu = 0;
 
     avrt::SceneRebuild rb;
+    avrt::HybridState hy;
+    bool last_hybrid = false;    // the last render was avrt_render_hybrid
 
     AvrtStats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_a = nullptr, ev_b = nullptr;
@@ -105,6 +129,17 @@ int upload_vec(avrt_ctx *c, DevBuf &b, const void *src, size_t bytes);
 // BLAS trees and triangle records for every mesh the instances use.
 int prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t ninst, bool *changed);
 int upload_mesh_tables(avrt_ctx *c);
+
+// Shared by the render translation units.
+dev::SceneDev scene_dev(const avrt_ctx *c);
+void camera_params(const AvrtConfig &cfg, FrameParams &fp);
+int frame_params(avrt_ctx *c, const AvrtConfig *cfg, FrameParams &fp);
+int validate_config(avrt_ctx *c, const AvrtConfig *cfg);
+int ensure_image(avrt_ctx *c, uint32_t w, uint32_t h);
+float elapsed(hipEvent_t a, hipEvent_t b);
+// Hybrid frame unit: sRGB table upload (once per context) and buffer release.
+int hybrid_init(avrt_ctx *c, const float *srgb_lut);
+void hybrid_release(avrt_ctx *c);
 
 } // namespace avrt
 

@@ -94,6 +94,37 @@ __device__ __forceinline__ void rt_sincos(float a, float *s, float *c)
 }
 
 // PCG-hash stream (ray_trace.comp.hybrid.hlsl:238-243).
+// exp(x) as the oracle defines it (HLSL leaves exp's precision open):
+// x = k ln2 + r with ln2 split in two (Cody-Waite), Taylor degree 7 on
+// |r| <= ln2/2, scaled by 2^k.
+__device__ __forceinline__ float rt_exp(float x)
+{
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return INFINITY;
+    const float k = rintf(x * 1.44269504f);
+    float r = x - k * 0.693145752f;
+    r = r - k * 1.42860677e-6f;
+    float p = 1.98412698e-4f;
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, static_cast<int>(k));
+}
+
+// pow(x, 32) as five squarings.
+__device__ __forceinline__ float pow32(float x)
+{
+    float y = x * x;
+    y = y * y;
+    y = y * y;
+    y = y * y;
+    return y * y;
+}
+
 struct Rng {
     uint32_t s;
     __device__ __forceinline__ float next()
@@ -206,6 +237,7 @@ struct SceneDev {
     const uint32_t *indices;
     const uint32_t *lights;
     const uchar4 *tex;
+    const AvrtMeshInstance *instances;
     uint32_t nobj;
 };
 
@@ -309,10 +341,12 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
 //   kLightGlass only lights and glass (kGeomLG), closest hit;
 //   kOpaqueAny  only the other objects, and the walk ends at the first hit
 //               closer than `term`: the any-hit occlusion test that skips
-//               glass and lights (Occluded, deferred.comp.hlsl:1115-1171).
+//               glass and lights (Occluded, deferred.comp.hlsl:1115-1171);
+//   kRaster     only what the rasterizer draws (kGeomRaster), closest hit;
+//   kNonRaster  everything else, closest hit (skipRasterized, :1064-1068).
 // Child masks in Node4::pad[0] prune subtrees that hold no object of the
 // requested class.
-constexpr int kAll = 0, kLightGlass = 1, kOpaqueAny = 2;
+constexpr int kAll = 0, kLightGlass = 1, kOpaqueAny = 2, kRaster = 3, kNonRaster = 4;
 
 template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll>
 struct Traversal {
@@ -390,7 +424,8 @@ struct Traversal {
             bool h2 = slab(LX.z, HX.z, LY.z, HY.z, LZ.z, HZ.z, ao, ainv, near, far, &t2) && R.z != kRefInvalid;
             bool h3 = slab(LX.w, HX.w, LY.w, HY.w, LZ.w, HZ.w, ao, ainv, near, far, &t3) && R.w != kRefInvalid;
             if (FILTER != kAll) {
-                const uint32_t m = np->pad[0] >> (FILTER == kLightGlass ? 0 : 4);
+                const uint32_t m =
+                    np->pad[0] >> (FILTER == kLightGlass ? 0 : FILTER == kOpaqueAny ? 4 : FILTER == kRaster ? 8 : 12);
                 h0 = h0 && (m & 1u);
                 h1 = h1 && (m & 2u);
                 h2 = h2 && (m & 4u);
@@ -486,6 +521,8 @@ struct Traversal {
             if ((flags & kGeomValid) == 0) continue;
             if (FILTER == kLightGlass && (flags & kGeomLG) == 0) continue;
             if (FILTER == kOpaqueAny && (flags & kGeomLG) != 0) continue;
+            if (FILTER == kRaster && (flags & kGeomRaster) == 0) continue;
+            if (FILTER == kNonRaster && (flags & kGeomRaster) != 0) continue;
             const uint32_t shape = flags & 0xFFu;
             float t = 0.0f, al = 0.0f, be = 0.0f;
             bool h = false;

@@ -1,0 +1,1081 @@
+// avrt_hybrid.hip - the hybrid frame of libavrt for gfx950: ray-cast
+// G-buffer, deferred lighting over the trace grid, temporal reprojection and
+// the edge-aware upsample (gbuffer.{vert,frag}.hlsl, deferred.comp.hlsl,
+// upsample.comp.hlsl; the frame sequence of renderer.cpp:1330-1583).
+//
+// One frame (avrt_render_hybrid):
+//   gbuffer   per full-res pixel: the jittered primary ray against what the
+//             rasterizer draws (meshes, raster-proxy boxes). Writes position +
+//             object slot (RGBA32F), normal + coverage (RGBA16F), albedo +
+//             view-dependence flag (RGBA8 UNORM) and depth (D32), the formats
+//             of gbuffer.frag.hlsl:96-122 and renderer.cpp:373-419.
+//   primary   per trace cell: traced texel (TracedTexel), seed, jittered
+//             camera ray, the G-buffer surface as the far bound
+//             (ResolvePrimary, deferred.comp.hlsl:1526-1548).
+//   resolve   closest non-rasterized hit in front of that surface; the cell's
+//             primary hit is the analytic one, else the G-buffer's, else sky.
+//   shade0    one lighting sample at the primary surface: sky / light colour,
+//             NEE shadow queries, first bounce (main :1950-1969, the head of
+//             TraceIndirect :1575-1601).
+//   bounces   extend -> shade -> occlusion queries, the loop of
+//             TraceIndirect (:1603-1646) with Occluded as the shadow test.
+//   finish    colour / samples. Reduced grid: demodulated into the reduced
+//             lighting target (:1971-1990). Full res: ReprojectHistory
+//             (:1810-1875), accumulation, geometry, display (:1992-2009).
+//   upsample  reduced grid only: 2x2 bilateral gather, remodulation,
+//             ReprojectHistory, display (upsample.comp.hlsl:280-390).
+// The lighting of a sample is summed exactly as the reference sums it: the
+// direct light of the primary surface into the cell colour, every later term
+// into a per-sample indirect sum that is added to the colour when the sample
+// ends (color += ClampedDirectLight; color += TraceIndirect).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/avrt.h"
+#include "avrt_ctx.h"
+#include "avrt_device.h"
+#include "avrt_internal.h"
+#include "avrt_shade.h"
+
+using namespace avrt;
+using namespace avrt::dev;
+
+namespace {
+
+constexpr uint32_t kSrcSky = 0, kSrcRaster = 1, kSrcAnalytic = 2;
+
+__device__ __forceinline__ unsigned short h_bits(float x) { return __half_as_ushort(__float2half_rn(x)); }
+__device__ __forceinline__ float h_val(unsigned short b) { return __half2float(__ushort_as_half(b)); }
+__device__ __forceinline__ float unorm_read(unsigned char b) { return static_cast<float>(b) / 255.0f; }
+
+__device__ __forceinline__ F3 jittered_dir(const FrameParams &fp, const HybridParams &hp, uint32_t x, uint32_t y)
+{
+    const F3 ps = ld3(fp.pixel0) + (static_cast<float>(x) + hp.jitter_x) * ld3(fp.pixelU) +
+                  (static_cast<float>(y) + hp.jitter_y) * ld3(fp.pixelV);
+    return normalize(ps - ld3(fp.source));
+}
+
+// TracedTexel (deferred.comp.hlsl:1742-1747 = upsample.comp.hlsl:205-210).
+__device__ __forceinline__ void cell_texel(const FrameParams &fp, const HybridParams &hp, uint32_t cx, uint32_t cy,
+                                           uint32_t *tx, uint32_t *ty)
+{
+    if (!hp.reduced) {
+        *tx = cx;
+        *ty = cy;
+        return;
+    }
+    const uint32_t k = hp.frame_index % 4u;
+    const float ox = (k == 1u || k == 2u) ? 0.5f : 0.0f;
+    const float oy = (k == 1u || k == 3u) ? 0.5f : 0.0f;
+    const float fx = floorf((static_cast<float>(cx) + ox) / hp.trace_scale);
+    const float fy = floorf((static_cast<float>(cy) + oy) / hp.trace_scale);
+    *tx = min(static_cast<uint32_t>(fx), fp.width - 1u);
+    *ty = min(static_cast<uint32_t>(fy), fp.height - 1u);
+}
+
+__device__ __forceinline__ F3 surface_albedo(const SceneDev &sc, const FrameParams &fp, const AvrtObject &ob,
+                                             const Surf &s)
+{
+    const F3 alb = f3(ob.r, ob.g, ob.b);
+    if (ob.textureID == AVRT_NO_TEXTURE) return alb;
+    const F3 texel = texture_color(sc, fp, ob.textureID, s.u, s.v);
+    return texel * lerp(f3(1.0f, 1.0f, 1.0f), alb, ob.textureTint);
+}
+
+// ---------------------------------------------------------------------------
+// G-buffer
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gbuffer(SceneDev sc, FrameParams fp, HybridParams hp, float4 *gpos,
+                                                    ushort4 *gnrm, uchar4 *galb, float *gdep)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t P = fp.width * fp.height;
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    uint32_t spill_mem[kStackSpill];
+    const uint32_t x = p % fp.width, y = p / fp.width;
+    const F3 src = ld3(fp.source);
+    const F3 d = jittered_dir(fp, hp, x, y);
+    Traversal<false, kStackLds, kBlock, kStackSpill, kRaster> tr;
+    if (tr.init(sc, src, d, 0.0f, 0.001f, 1000000.0f, lds_stack, spill_mem)) {
+        while (tr.step(sc)) {
+        }
+    }
+    if (tr.hit.obj == ~0u) {
+        gpos[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        gnrm[p] = make_ushort4(0, 0, 0, 0);
+        galb[p] = make_uchar4(0, 0, 0, 0);
+        gdep[p] = 1.0f;
+        return;
+    }
+    const AvrtObject ob = sc.objects[tr.hit.obj];
+    Surf s;
+    hit_attributes(sc, ob, tr.hit, src, d, 0.0f, s);
+    // the raster normal is the surface's own, not turned toward the viewer
+    const F3 n = s.face ? s.normal : -s.normal;
+    const F3 alb = surface_albedo(sc, fp, ob, s);
+    const bool view_dep = ob.colorType == AVRT_METAL || ob.colorType == AVRT_DIELECTRIC;
+    gpos[p] = make_float4(s.pos.x, s.pos.y, s.pos.z, static_cast<float>(tr.hit.obj));
+    gnrm[p] = make_ushort4(h_bits(n.x), h_bits(n.y), h_bits(n.z), h_bits(1.0f));
+    galb[p] = make_uchar4(unorm8(alb.x), unorm8(alb.y), unorm8(alb.z), view_dep ? 0 : 255);
+    const float zv = dot(ld3(hp.view_w), s.pos) - hp.view_wd;
+    gdep[p] = (hp.proj_m22 * zv + hp.proj_m23) / (-zv);
+}
+
+// ---------------------------------------------------------------------------
+// Deferred pass
+// ---------------------------------------------------------------------------
+struct GView {
+    const float4 *pos;
+    const ushort4 *nrm;
+    const uchar4 *alb;
+};
+
+struct CellState {
+    float4 *o;       // primary ray origin.xyz, time
+    float4 *d;       // primary ray direction.xyz, cell (bits)
+    float *far;      // G-buffer bound of the resolve query
+    uint4 *hit;      // resolved hit: t, slot, triangle, alpha
+    float4 *hb;      // beta, source (bits), final (bits: sky or light, no samples)
+    uint32_t *rng;   // RNG state between samples
+    float4 *lcol;    // running colour
+    float4 *lind;    // indirect sum of the current sample
+};
+
+struct Queries {
+    float4 *o;  // origin.xyz, time
+    float4 *d;  // direction.xyz, NEE contribution slot (bits)
+    float *far; // Occluded far bound
+};
+
+__global__ __launch_bounds__(kBlock) void k_def_primary(FrameParams fp, HybridParams hp, GView g, CellState cs)
+{
+    const uint32_t n = hp.trace_w * hp.trace_h;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    uint32_t ix, iy;
+    cell_texel(fp, hp, c % hp.trace_w, c / hp.trace_w, &ix, &iy);
+    const uint32_t W = fp.width, H = fp.height;
+    Rng rng{ix + iy * W + hp.frame_index * W * H + 1u};
+    const F3 src = ld3(fp.source);
+    const F3 d = jittered_dir(fp, hp, ix, iy);
+    const float time = rng.next();
+    const uint32_t pix = iy * W + ix;
+    float far = 1000000.0f;
+    if (h_val(g.nrm[pix].w) >= 0.5f) {
+        const float4 P = g.pos[pix];
+        far = length(f3(P.x, P.y, P.z) - src);
+    }
+    cs.o[c] = make_float4(src.x, src.y, src.z, time);
+    cs.d[c] = make_float4(d.x, d.y, d.z, __uint_as_float(c));
+    cs.far[c] = far;
+    cs.rng[c] = rng.s;
+    cs.lcol[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_def_resolve(FrameParams fp, HybridParams hp, GView g, const uint4 *hitA,
+                                                        const float *hitB, CellState cs)
+{
+    const uint32_t n = hp.trace_w * hp.trace_h;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    uint32_t ix, iy;
+    cell_texel(fp, hp, c % hp.trace_w, c / hp.trace_w, &ix, &iy);
+    const uint4 H = hitA[c];
+    uint32_t src = kSrcAnalytic;
+    if (H.y == ~0u) src = (h_val(g.nrm[iy * fp.width + ix].w) >= 0.5f) ? kSrcRaster : kSrcSky;
+    cs.hit[c] = H;
+    cs.hb[c] = make_float4(hitB[c], __uint_as_float(src), __uint_as_float(0u), 0.0f);
+}
+
+// The primary surface of cell c as the reference's Hit: an analytic hit with
+// its own material, or the G-buffer's surface (HitFromGBuffer,
+// deferred.comp.hlsl:1452-1493) with the albedo folded in and the texture
+// cleared. Returns the source.
+__device__ uint32_t primary_surface(const SceneDev &sc, const FrameParams &fp, const HybridParams &hp, GView g,
+                                    const CellState &cs, uint32_t c, F3 *o, F3 *d, float *time, AvrtObject *ob,
+                                    Surf *s, F3 *salb, F3 *alb_flat)
+{
+    const float4 O = cs.o[c], D = cs.d[c];
+    *o = f3(O.x, O.y, O.z);
+    *d = f3(D.x, D.y, D.z);
+    *time = O.w;
+    const float4 HB = cs.hb[c];
+    const uint32_t src = __float_as_uint(HB.y);
+    s->pos = f3(0.0f, 0.0f, 0.0f);
+    s->normal = f3(0.0f, 0.0f, 0.0f);
+    s->u = 0.0f;
+    s->v = 0.0f;
+    s->face = false;
+    if (src == kSrcAnalytic) {
+        const uint4 H = cs.hit[c];
+        HitRec h;
+        h.t = __uint_as_float(H.x);
+        h.obj = H.y;
+        h.prim = H.z;
+        h.alpha = __uint_as_float(H.w);
+        h.beta = HB.x;
+        *ob = sc.objects[h.obj];
+        hit_attributes(sc, *ob, h, *o, *d, *time, *s);
+        *alb_flat = f3(ob->r, ob->g, ob->b);
+        *salb = surface_albedo(sc, fp, *ob, *s);
+    } else if (src == kSrcRaster) {
+        uint32_t ix, iy;
+        cell_texel(fp, hp, c % hp.trace_w, c / hp.trace_w, &ix, &iy);
+        const uint32_t pix = iy * fp.width + ix;
+        const float4 P = g.pos[pix];
+        const ushort4 N = g.nrm[pix];
+        const uchar4 A = g.alb[pix];
+        *ob = sc.objects[static_cast<uint32_t>(P.w + 0.5f)];
+        s->pos = f3(P.x, P.y, P.z);
+        s->normal = normalize(f3(h_val(N.x), h_val(N.y), h_val(N.z)));
+        s->face = dot(*d, s->normal) < 0.0f;
+        if (!s->face) s->normal = -s->normal;
+        *alb_flat = f3(unorm_read(A.x), unorm_read(A.y), unorm_read(A.z));
+        *salb = *alb_flat;
+    }
+    return src;
+}
+
+// NEE of one surface: per light, a bounded any-hit query (Occluded) or, when
+// there are more lights than the query wavefront carries, the walks inline.
+struct NeeOut {
+    bool want;
+    F3 lc[kWaveLights];
+    bool lray[kWaveLights];
+    F3 ldir[kWaveLights];
+    float lfar[kWaveLights];
+};
+
+template <bool WAVE_NEE>
+__device__ void def_scatter(const SceneDev &sc, const FrameParams &fp, const AvrtObject &ob, const Surf &s, F3 d,
+                            float time, F3 salb, F3 tp, bool first, Rng &rng, uint32_t *lds_stack, NeeOut &nee,
+                            F3 *inline_direct, bool *cont, F3 *no, F3 *nd, F3 *ntp, uint32_t bounce)
+{
+    nee.want = false;
+    *inline_direct = f3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < kWaveLights; i++) {
+        nee.lray[i] = false;
+        nee.lc[i] = f3(0.0f, 0.0f, 0.0f);
+        nee.ldir[i] = f3(0.0f, 0.0f, 0.0f);
+        nee.lfar[i] = 0.0f;
+    }
+    if (ob.colorType == AVRT_LAMBERTIAN && fp.num_lights > 0) {
+        nee.want = true;
+        if (WAVE_NEE) {
+            for (uint32_t i = 0; i < kWaveLights; i++) {
+                if (i >= fp.num_lights) break;
+                F3 lp;
+                float dist;
+                nee.lray[i] = nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lp, &nee.ldir[i], &dist, &nee.lc[i]);
+                nee.lfar[i] = dist - 0.001f;
+                if (!nee.lray[i]) nee.lc[i] = f3(0.0f, 0.0f, 0.0f);
+            }
+        } else {
+            uint32_t spill_mem[kStackSpill];
+            F3 sum = f3(0.0f, 0.0f, 0.0f);
+            const F3 so = s.pos + (s.normal * 0.001f);
+            for (uint32_t i = 0; i < fp.num_lights; i++) {
+                F3 lp, dir, lc;
+                float dist;
+                if (!nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lp, &dir, &dist, &lc)) continue;
+                Traversal<false, kStackLds, kBlock, kStackSpill, kOpaqueAny> tr;
+                bool occluded = false;
+                if (tr.init(sc, so, dir, time, 0.001f, dist - 0.001f, lds_stack, spill_mem)) {
+                    tr.term = INFINITY;
+                    while (tr.step(sc)) {
+                    }
+                    occluded = tr.hit.obj != ~0u;
+                }
+                if (!occluded) sum = sum + lc;
+            }
+            const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
+            if (luma > 8.0f) sum = sum * (8.0f / luma);
+            *inline_direct = tp * sum;
+        }
+    }
+    *cont = false;
+    F3 b;
+    if (get_bounce(ob, d, s, rng, &b)) {
+        const F3 n2 = normalize(b);
+        const float sign = (dot(n2, s.normal) < 0.0f) ? -1.0f : 1.0f;
+        const F3 o2 = s.pos + s.normal * (0.001f * sign);
+        F3 t2 = tp;
+        if (ob.colorType != AVRT_DIELECTRIC) t2 = first ? salb : tp * salb;
+        bool alive = true;
+        if (!first) {
+            float p = fmaxf(t2.x, fmaxf(t2.y, t2.z));
+            p = fminf(fmaxf(p, 0.1f), 1.0f);
+            if (rng.next() > p) alive = false;
+            else t2 = f3(t2.x / p, t2.y / p, t2.z / p);
+        }
+        if (alive) {
+            *no = o2;
+            *nd = n2;
+            *ntp = t2;
+            *cont = bounce + 1 < fp.depth;
+        }
+    }
+}
+
+// Queue the NEE record and its shadow queries (wave NEE only).
+__device__ __forceinline__ void emit_nee(const NeeOut &nee, const Surf &s, float time, F3 tp, uint32_t pid,
+                                         uint32_t nl, Queries q, NeeBufs rec, uint32_t *counters)
+{
+    const uint32_t r = wave_append(nee.want, &counters[1]);
+    if (nee.want) {
+        rec.tp[r] = make_float4(tp.x, tp.y, tp.z, __uint_as_float(pid));
+        for (uint32_t i = 0; i < kWaveLights; i++)
+            if (i < nl) rec.contrib[r * nl + i] = make_float4(nee.lc[i].x, nee.lc[i].y, nee.lc[i].z, 0.0f);
+    }
+    const F3 so = s.pos + (s.normal * 0.001f);
+    for (uint32_t i = 0; i < kWaveLights; i++) {
+        const bool emit = nee.want && nee.lray[i];
+        const uint32_t k = wave_append(emit, &counters[2]);
+        if (emit) {
+            q.o[k] = make_float4(so.x, so.y, so.z, time);
+            q.d[k] = make_float4(nee.ldir[i].x, nee.ldir[i].y, nee.ldir[i].z, __uint_as_float(r * nl + i));
+            q.far[k] = nee.lfar[i];
+        }
+    }
+}
+
+__device__ __forceinline__ void add_to(float4 *buf, uint32_t i, F3 v)
+{
+    float4 L = buf[i];
+    L.x = L.x + v.x;
+    L.y = L.y + v.y;
+    L.z = L.z + v.z;
+    buf[i] = L;
+}
+
+// One lighting sample at each cell's primary surface.
+template <bool WAVE_NEE>
+__global__ __launch_bounds__(kBlock) void k_def_shade0(SceneDev sc, FrameParams fp, HybridParams hp, GView g,
+                                                       CellState cs, uint32_t sample, PathBufs out, Queries q,
+                                                       NeeBufs rec, uint32_t *counters)
+{
+    __shared__ uint32_t lds_stack[WAVE_NEE ? 1 : kStackLds * kBlock];
+    const uint32_t n = hp.trace_w * hp.trace_h;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    bool active = c < n;
+    bool cont = false;
+    NeeOut nee;
+    nee.want = false;
+    for (uint32_t i = 0; i < kWaveLights; i++) nee.lray[i] = false;
+    Surf s;
+    s.pos = f3(0.0f, 0.0f, 0.0f);
+    s.normal = f3(0.0f, 0.0f, 0.0f);
+    float time = 0.0f;
+    F3 no = f3(0, 0, 0), nd = f3(0, 0, 0), ntp = f3(0, 0, 0);
+    uint32_t rs = 0;
+    if (active) {
+        F3 o, d, salb, alb_flat;
+        AvrtObject ob;
+        const uint32_t src = primary_surface(sc, fp, hp, g, cs, c, &o, &d, &time, &ob, &s, &salb, &alb_flat);
+        const bool fin = src == kSrcSky || ob.colorType == AVRT_DIFFUSE_LIGHT;
+        if (fin) {
+            active = false;
+            if (sample == 0) {
+                const F3 col = (src == kSrcSky) ? ld3(fp.sky) : ob.emission * alb_flat;
+                cs.lcol[c] = make_float4(col.x, col.y, col.z, 0.0f);
+                float4 HB = cs.hb[c];
+                HB.z = __uint_as_float(1u);
+                cs.hb[c] = HB;
+            }
+        } else {
+            cs.lind[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            Rng rng{cs.rng[c]};
+            F3 direct;
+            def_scatter<WAVE_NEE>(sc, fp, ob, s, d, time, salb, f3(1.0f, 1.0f, 1.0f), true, rng, lds_stack, nee, &direct,
+                                  &cont, &no, &nd, &ntp, 0);
+            if (!WAVE_NEE && nee.want) add_to(cs.lcol, c, direct);
+            rs = rng.s;
+            if (!cont) cs.rng[c] = rs;
+        }
+    }
+    if (WAVE_NEE) {
+        if (!active) nee.want = false;
+        emit_nee(nee, s, time, f3(1.0f, 1.0f, 1.0f), c, fp.num_lights, q, rec, counters);
+    }
+    const uint32_t slot = wave_append(cont, &counters[0]);
+    if (cont) {
+        out.o[slot] = make_float4(no.x, no.y, no.z, time);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(c));
+        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(rs));
+    }
+}
+
+// Bounces 1 .. Depth-1 of TraceIndirect.
+template <bool WAVE_NEE>
+__global__ __launch_bounds__(kBlock) void k_def_shade(SceneDev sc, FrameParams fp, uint32_t n, uint32_t bounce,
+                                                      PathBufs in, const uint4 *__restrict__ hitA,
+                                                      const float *__restrict__ hitB, CellState cs, PathBufs out,
+                                                      Queries q, NeeBufs rec, uint32_t *counters)
+{
+    __shared__ uint32_t lds_stack[WAVE_NEE ? 1 : kStackLds * kBlock];
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    bool cont = false;
+    NeeOut nee;
+    nee.want = false;
+    for (uint32_t i = 0; i < kWaveLights; i++) nee.lray[i] = false;
+    Surf s;
+    s.pos = f3(0.0f, 0.0f, 0.0f);
+    s.normal = f3(0.0f, 0.0f, 0.0f);
+    float time = 0.0f;
+    uint32_t pid = 0, rs = 0;
+    F3 tp = f3(0, 0, 0), no = f3(0, 0, 0), nd = f3(0, 0, 0), ntp = f3(0, 0, 0);
+    if (k < n) {
+        const float4 O = in.o[k], D = in.d[k], T = in.t[k];
+        const uint4 HA = hitA[k];
+        const F3 o = f3(O.x, O.y, O.z), d = f3(D.x, D.y, D.z);
+        tp = f3(T.x, T.y, T.z);
+        time = O.w;
+        pid = __float_as_uint(D.w);
+        Rng rng{__float_as_uint(T.w)};
+        if (HA.y == ~0u) {
+            add_to(cs.lind, pid, tp * ld3(fp.sky));
+        } else {
+            HitRec h;
+            h.t = __uint_as_float(HA.x);
+            h.obj = HA.y;
+            h.prim = HA.z;
+            h.alpha = __uint_as_float(HA.w);
+            h.beta = hitB[k];
+            const AvrtObject ob = sc.objects[h.obj];
+            hit_attributes(sc, ob, h, o, d, time, s);
+            if (ob.colorType == AVRT_DIFFUSE_LIGHT) {
+                add_to(cs.lind, pid, tp * (ob.emission * f3(ob.r, ob.g, ob.b)));
+            } else {
+                const F3 salb = surface_albedo(sc, fp, ob, s);
+                F3 direct;
+                def_scatter<WAVE_NEE>(sc, fp, ob, s, d, time, salb, tp, false, rng, lds_stack, nee, &direct, &cont, &no,
+                                      &nd, &ntp, bounce);
+                if (!WAVE_NEE && nee.want) add_to(cs.lind, pid, direct);
+            }
+        }
+        rs = rng.s;
+        if (!cont) cs.rng[pid] = rs;
+    }
+    if (WAVE_NEE) emit_nee(nee, s, time, tp, pid, fp.num_lights, q, rec, counters);
+    const uint32_t slot = wave_append(cont, &counters[0]);
+    if (cont) {
+        out.o[slot] = make_float4(no.x, no.y, no.z, time);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(pid));
+        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(rs));
+    }
+}
+
+// An occluded light contributes nothing.
+__global__ __launch_bounds__(kBlock) void k_occl_apply(uint32_t n, const float4 *__restrict__ qd,
+                                                       const uint4 *__restrict__ hitA, float4 *contrib)
+{
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    if (hitA[k].y != ~0u) contrib[__float_as_uint(qd[k].w)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_def_sample_end(uint32_t n, CellState cs)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    if (__float_as_uint(cs.hb[c].z) != 0u) return;
+    const float4 I = cs.lind[c];
+    add_to(cs.lcol, c, f3(I.x, I.y, I.z));
+}
+
+__device__ __forceinline__ uchar4 display_of(F3 a, float e)
+{
+    return make_uchar4(unorm8(to_srgb(aces(a.x * e))), unorm8(to_srgb(aces(a.y * e))), unorm8(to_srgb(aces(a.z * e))),
+                       255);
+}
+
+// CurrentRotation / PreviousRotation / PreviousPosition (deferred.comp.hlsl
+// :1759-1793 = upsample.comp.hlsl:124-159).
+__device__ F3 previous_position(const SceneDev &sc, const AvrtObject &o, F3 p)
+{
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f, rw = 1.0f;
+    if (o.shapeType == AVRT_SHAPE_MESH) {
+        const AvrtMeshInstance in = sc.instances[o.instanceIndex];
+        rx = in.qx;
+        ry = in.qy;
+        rz = in.qz;
+        rw = in.qw;
+    } else if (o.shapeType == AVRT_SHAPE_SPHERE || o.shapeType == AVRT_SHAPE_BOX) {
+        const float h = 0.5f * o.rotation;
+        rt_sincos(h, &ry, &rw);
+    }
+    const F3 local = quat_rotate(-rx, -ry, -rz, rw, p - f3(o.x, o.y, o.z));
+    const F3 pv = f3(o.prevRotX, o.prevRotY, o.prevRotZ);
+    const float pw = sqrtf(fmaxf(0.0f, 1.0f - dot(pv, pv)));
+    return quat_rotate(pv.x, pv.y, pv.z, pw, local) + f3(o.x2, o.y2, o.z2);
+}
+
+// ReprojectHistory (deferred.comp.hlsl:1810-1875 = upsample.comp.hlsl:218-278).
+__device__ void reproject(const FrameParams &fp, const HybridParams &hp, uint32_t px, uint32_t py, F3 color,
+                          bool surface, F3 position, F3 prev_position, F3 normal, bool view_dep,
+                          const float4 *prev_image, const float4 *prev_geom, F3 *result, float *count)
+{
+    const int W = static_cast<int>(fp.width), H = static_cast<int>(fp.height);
+    int hx = static_cast<int>(px), hy = static_cast<int>(py);
+    bool valid = hp.history_reset == 0u;
+    const F3 src = ld3(fp.source);
+    if (valid && hp.camera_moved != 0u) {
+        valid = false;
+        if (surface) {
+            const float *m = hp.prev_view_proj;
+            const F3 q = prev_position;
+            const float cx = m[0] * q.x + m[1] * q.y + m[2] * q.z + m[3] * 1.0f;
+            const float cy = m[4] * q.x + m[5] * q.y + m[6] * q.z + m[7] * 1.0f;
+            const float cw = m[12] * q.x + m[13] * q.y + m[14] * q.z + m[15] * 1.0f;
+            if (cw > 0.0f) {
+                const float u = (cx / cw) * 0.5f + 0.5f;
+                const float v = (cy / cw) * -0.5f + 0.5f;
+                hx = static_cast<int>(floorf(u * static_cast<float>(W)));
+                hy = static_cast<int>(floorf(v * static_cast<float>(H)));
+                if (hx >= 0 && hy >= 0 && hx < W && hy < H) {
+                    const float4 pg = prev_geom[static_cast<size_t>(hy) * W + hx];
+                    const F3 prev = f3(pg.x, pg.y, pg.z);
+                    const float plane = fabsf(dot(prev - prev_position, normal));
+                    const float tol = 0.01f * length(position - src) + 0.01f;
+                    const float footprint = length(position - src) * 2.0f * hp.tan_half_fov / static_cast<float>(H);
+                    const float facing = fmaxf(fabsf(dot(normal, normalize(src - position))), 0.33f);
+                    const float drift = length(prev - prev_position);
+                    valid = pg.w > 0.0f && plane < tol && drift < fmaxf(2.5f * footprint / facing, 0.01f);
+                }
+            }
+        }
+    }
+    if (!valid) {
+        *count = 1.0f;
+        *result = color;
+        return;
+    }
+    const size_t hidx = static_cast<size_t>(hy) * W + hx;
+    const float pc = fabsf(prev_geom[hidx].w);
+    const float moving = view_dep ? fminf(hp.history_cap, 4.0f) : hp.history_cap;
+    const float cap = (hp.camera_moved != 0u) ? moving : 1e7f;
+    *count = fminf(pc + 1.0f, cap);
+    const float4 pi = prev_image[hidx];
+    *result = lerp(f3(pi.x, pi.y, pi.z), color, 1.0f / *count);
+}
+
+struct History {
+    const float4 *prev_img, *prev_geom;
+    float4 *img, *geom;
+};
+
+__device__ __forceinline__ float store_round(float x, uint32_t fp16) { return fp16 ? round_half(x) : x; }
+
+__global__ __launch_bounds__(kBlock) void k_def_finish(SceneDev sc, FrameParams fp, HybridParams hp, GView g,
+                                                       CellState cs, History hist, float4 *lred, uchar4 *display)
+{
+    const uint32_t n = hp.trace_w * hp.trace_h;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    F3 o, d, salb = f3(0, 0, 0), alb_flat = f3(0, 0, 0);
+    float time;
+    AvrtObject ob;
+    ob.colorType = AVRT_LAMBERTIAN;
+    Surf s;
+    const uint32_t src = primary_surface(sc, fp, hp, g, cs, c, &o, &d, &time, &ob, &s, &salb, &alb_flat);
+    const float4 L = cs.lcol[c];
+    F3 color = f3(L.x, L.y, L.z);
+    if (__float_as_uint(cs.hb[c].z) == 0u) color = color / static_cast<float>(hp.spf);
+    if (hp.reduced) {
+        const bool demod = src == kSrcRaster && ob.colorType != AVRT_DIELECTRIC;
+        F3 a = f3(1.0f, 1.0f, 1.0f);
+        if (demod) a = f3(fmaxf(salb.x, 0.01f), fmaxf(salb.y, 0.01f), fmaxf(salb.z, 0.01f));
+        const F3 r = f3(color.x / a.x, color.y / a.y, color.z / a.z);
+        lred[c] = make_float4(store_round(r.x, fp.fp16), store_round(r.y, fp.fp16), store_round(r.z, fp.fp16),
+                              demod ? 1.0f : 0.0f);
+        return;
+    }
+    const uint32_t px = c % fp.width, py = c / fp.width;
+    const bool surface = src != kSrcSky;
+    const bool view_dep = surface && (ob.colorType == AVRT_METAL || ob.colorType == AVRT_DIELECTRIC);
+    const F3 prev_pos = surface ? previous_position(sc, ob, s.pos) : s.pos;
+    F3 acc;
+    float count;
+    reproject(fp, hp, px, py, color, surface, s.pos, prev_pos, s.normal, view_dep, hist.prev_img, hist.prev_geom, &acc,
+              &count);
+    // the target stores the accumulation (RGBA16F when emulated); the display
+    // grades the value before storage, as the reference does
+    hist.img[c] = make_float4(store_round(acc.x, fp.fp16), store_round(acc.y, fp.fp16), store_round(acc.z, fp.fp16), 1.0f);
+    hist.geom[c] = make_float4(s.pos.x, s.pos.y, s.pos.z, surface ? count : -count);
+    display[c] = display_of(acc, fp.exposure);
+}
+
+// ---------------------------------------------------------------------------
+// Upsample (upsample.comp.hlsl:280-390)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_upsample(SceneDev sc, FrameParams fp, HybridParams hp, GView g,
+                                                     const float4 *lred, History hist, uchar4 *display)
+{
+    const uint32_t P = fp.width * fp.height;
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t px = p % fp.width, py = p / fp.width;
+    const F3 src = ld3(fp.source);
+    const int cmx = static_cast<int>(hp.trace_w) - 1, cmy = static_cast<int>(hp.trace_h) - 1;
+    const ushort4 NP = g.nrm[p];
+    const F3 normal_p = f3(h_val(NP.x), h_val(NP.y), h_val(NP.z));
+    const bool covered_p = h_val(NP.w) > 0.5f;
+    const float4 GP = g.pos[p];
+    const F3 position_p = f3(GP.x, GP.y, GP.z);
+    const float depth_p = length(position_p - src);
+    const uchar4 AP = g.alb[p];
+    const F3 albedo_p = f3(unorm_read(AP.x), unorm_read(AP.y), unorm_read(AP.z));
+    const float cpx = (static_cast<float>(px) + 0.5f) * hp.trace_scale - 0.5f;
+    const float cpy = (static_cast<float>(py) + 0.5f) * hp.trace_scale - 0.5f;
+    const int bx = static_cast<int>(floorf(cpx)), by = static_cast<int>(floorf(cpy));
+    F3 sum = f3(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    F3 nearest = f3(0.0f, 0.0f, 0.0f);
+    float nearest_d = 1e30f;
+    for (int j = 0; j < 2; j++) {
+        for (int i = 0; i < 2; i++) {
+            const int cx = min(max(bx + i, 0), cmx), cy = min(max(by + j, 0), cmy);
+            uint32_t tx, ty;
+            cell_texel(fp, hp, static_cast<uint32_t>(cx), static_cast<uint32_t>(cy), &tx, &ty);
+            const float4 lt = lred[static_cast<size_t>(cy) * hp.trace_w + cx];
+            const uint32_t t = ty * fp.width + tx;
+            const ushort4 NT = g.nrm[t];
+            const bool covered_t = h_val(NT.w) > 0.5f;
+            F3 alb = albedo_p;
+            if (!covered_p) {
+                const uchar4 AT = g.alb[t];
+                alb = f3(unorm_read(AT.x), unorm_read(AT.y), unorm_read(AT.z));
+            }
+            const F3 lrgb = f3(lt.x, lt.y, lt.z);
+            const F3 radiance = (lt.w > 0.5f) ? lrgb * alb : lrgb;
+            const float ox = static_cast<float>(tx) - static_cast<float>(px);
+            const float oy = static_cast<float>(ty) - static_cast<float>(py);
+            const float d2 = ox * ox + oy * oy;
+            if (d2 < nearest_d) {
+                nearest_d = d2;
+                nearest = radiance;
+            }
+            float w = rt_exp((-d2 * hp.trace_scale) * hp.trace_scale);
+            if (covered_t != covered_p) {
+                w = 0.0f;
+            } else if (covered_p) {
+                const float4 GT = g.pos[t];
+                const float depth_t = length(f3(GT.x, GT.y, GT.z) - src);
+                w = w * rt_exp(-fabsf(depth_p - depth_t) / (0.01f * depth_p + 0.001f));
+                const F3 normal_t = f3(h_val(NT.x), h_val(NT.y), h_val(NT.z));
+                w = w * pow32(saturate(dot(normal_p, normal_t)));
+            }
+            sum = sum + w * radiance;
+            wsum = wsum + w;
+        }
+    }
+    const F3 color = (wsum > 1e-4f) ? sum / wsum : nearest;
+    const bool view_dep = covered_p && unorm_read(AP.w) < 0.5f;
+    const F3 prev_pos =
+        covered_p ? previous_position(sc, sc.objects[static_cast<uint32_t>(GP.w)], position_p) : position_p;
+    F3 acc;
+    float count;
+    reproject(fp, hp, px, py, color, covered_p, position_p, prev_pos, normal_p, view_dep, hist.prev_img, hist.prev_geom,
+              &acc, &count);
+    hist.img[p] = make_float4(store_round(acc.x, fp.fp16), store_round(acc.y, fp.fp16), store_round(acc.z, fp.fp16), 1.0f);
+    hist.geom[p] = make_float4(position_p.x, position_p.y, position_p.z, covered_p ? count : -count);
+    display[p] = display_of(acc, fp.exposure);
+}
+
+__global__ __launch_bounds__(kBlock) void k_hy_display_only(FrameParams fp, const float4 *image, uchar4 *display)
+{
+    const uint32_t P = fp.width * fp.height;
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    const float4 a = image[p];
+    display[p] = display_of(f3(a.x, a.y, a.z), fp.exposure);
+}
+
+// ---------------------------------------------------------------------------
+// Debug views (DebugColour, deferred.comp.hlsl:1652-1725), full resolution.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ F3 material_colour(uint32_t ct)
+{
+    switch (ct) {
+    case AVRT_LAMBERTIAN: return f3(0.55f, 0.55f, 0.55f);
+    case AVRT_METAL: return f3(1.0f, 0.75f, 0.1f);
+    case AVRT_DIELECTRIC: return f3(0.1f, 0.85f, 1.0f);
+    case AVRT_DIFFUSE_LIGHT: return f3(1.0f, 1.0f, 1.0f);
+    case AVRT_ISOTROPIC: return f3(0.7f, 0.3f, 1.0f);
+    default: return f3(1.0f, 0.0f, 1.0f);
+    }
+}
+
+__device__ __forceinline__ float frac(float x) { return x - floorf(x); }
+
+__global__ __launch_bounds__(kBlock) void k_def_debug(SceneDev sc, FrameParams fp, HybridParams hp, GView g,
+                                                      CellState cs, const float *gdep, uchar4 *display)
+{
+    const uint32_t n = hp.trace_w * hp.trace_h;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const ushort4 N = g.nrm[c];
+    const bool covered = h_val(N.w) >= 0.5f;
+    const F3 bg = f3(0.0f, 0.0f, 0.0f);
+    F3 col = bg;
+    const uint32_t view = hp.debug_view;
+    const F3 src = ld3(fp.source);
+    if (view == 1u) {
+        const uchar4 A = g.alb[c];
+        if (covered) col = f3(unorm_read(A.x), unorm_read(A.y), unorm_read(A.z));
+    } else if (view == 2u) {
+        if (covered) col = f3(h_val(N.x), h_val(N.y), h_val(N.z)) * 0.5f + f3(0.5f, 0.5f, 0.5f);
+    } else if (view == 3u) {
+        const float4 P = g.pos[c];
+        if (covered) col = f3(frac(P.x / 100.0f), frac(P.y / 100.0f), frac(P.z / 100.0f));
+    } else if (view == 6u) {
+        if (!covered) {
+            col = f3(0.0f, 0.0f, 0.2f);
+        } else {
+            const float4 P = g.pos[c];
+            const F3 stored = f3(P.x, P.y, P.z);
+            const float4 D = cs.d[c];
+            const F3 dir = f3(D.x, D.y, D.z);
+            const F3 fwd = -ld3(hp.view_w);
+            const float dd = gdep[c];
+            const float vz = (hp.z_near_far[0] * hp.z_near_far[1]) /
+                             (hp.z_near_far[1] + dd * (hp.z_near_far[0] - hp.z_near_far[1]));
+            const float t = vz / dot(dir, fwd);
+            const F3 rebuilt = src + dir * t;
+            const float rel = length(rebuilt - stored) / fmaxf(length(stored - src), 1e-4f);
+            col = f3(rel * 100.0f, rel * 100.0f, rel * 100.0f);
+        }
+    } else {
+        F3 o, d, salb, alb_flat;
+        float time;
+        AvrtObject ob;
+        Surf s;
+        const uint32_t srcid = primary_surface(sc, fp, hp, g, cs, c, &o, &d, &time, &ob, &s, &salb, &alb_flat);
+        if (view == 4u) {
+            col = srcid != kSrcSky ? material_colour(ob.colorType) : bg;
+        } else {
+            if (srcid == kSrcRaster) col = f3(0.1f, 0.8f, 0.2f);
+            else if (srcid == kSrcAnalytic) col = f3(0.85f, 0.15f, 0.1f);
+            else col = f3(0.1f, 0.15f, 0.5f);
+        }
+    }
+    display[c] = make_uchar4(unorm8(col.x), unorm8(col.y), unorm8(col.z), 255);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace avrt {
+
+int hybrid_init(avrt_ctx *c, const float *srgb_lut)
+{
+    (void)c;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), srgb_lut, 256 * sizeof(float)) == hipSuccess ? AVRT_OK : AVRT_E_DEVICE;
+}
+
+void hybrid_release(avrt_ctx *c)
+{
+    HybridState &h = c->hy;
+    DevBuf *bufs[] = {&h.gpos, &h.gnrm, &h.galb, &h.gdep, &h.img[0], &h.img[1], &h.geom[0], &h.geom[1], &h.lred,
+                      &h.pr_o, &h.pr_d, &h.pr_far, &h.pr_hit, &h.pr_hb, &h.rng, &h.lcol, &h.lind, &h.po[0], &h.po[1],
+                      &h.pd[0], &h.pd[1], &h.pt[0], &h.pt[1], &h.hitA, &h.hitB, &h.so, &h.sd, &h.sfar, &h.nee_tp,
+                      &h.nee_contrib, &h.counters};
+    for (DevBuf *b : bufs) release(*b);
+}
+
+} // namespace avrt
+
+namespace {
+
+inline uint32_t blocks_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+// Host values of the hybrid uniforms (renderer.cpp:1366-1423, mat4.h:56-94).
+void hybrid_params(const AvrtConfig &cfg, HybridParams &hp)
+{
+    std::memset(&hp, 0, sizeof(hp));
+    hp.jitter_x = cfg.jitter_x;
+    hp.jitter_y = cfg.jitter_y;
+    hp.trace_scale = cfg.traceScale > 0.0f && cfg.traceScale < 1.0f ? cfg.traceScale : 1.0f;
+    hp.frame_index = cfg.frameIndex;
+    hp.history_reset = cfg.historyReset;
+    hp.camera_moved = cfg.cameraMoved;
+    hp.history_cap = cfg.historyCap;
+    hp.tan_half_fov = std::tan((cfg.fov / 2.0f) * 0.0174532925f);
+    hp.reduced = hp.trace_scale < 1.0f ? 1u : 0u;
+    hp.trace_w = hp.reduced ? static_cast<uint32_t>(std::ceil(static_cast<float>(cfg.width) * hp.trace_scale)) : cfg.width;
+    hp.trace_h = hp.reduced ? static_cast<uint32_t>(std::ceil(static_cast<float>(cfg.height) * hp.trace_scale)) : cfg.height;
+    const uint32_t b = cfg.batches ? cfg.batches : 1u;
+    hp.spf = std::max(cfg.samples / b, 1u);
+    hp.debug_view = cfg.debugView;
+    std::memcpy(hp.prev_view_proj, cfg.prevViewProj, sizeof(hp.prev_view_proj));
+    // w = normalize(eye - target), the basis of camera_params and LookAtRH
+    float w[3] = {cfg.source_x - cfg.target_x, cfg.source_y - cfg.target_y, cfg.source_z - cfg.target_z};
+    const float inv = 1.0f / std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int k = 0; k < 3; k++) w[k] = w[k] * inv;
+    std::memcpy(hp.view_w, w, sizeof(w));
+    hp.view_wd = w[0] * cfg.source_x + w[1] * cfg.source_y + w[2] * cfg.source_z;
+    hp.proj_m22 = cfg.zFar / (cfg.zNear - cfg.zFar);
+    hp.proj_m23 = (cfg.zNear * cfg.zFar) / (cfg.zNear - cfg.zFar);
+    hp.z_near_far[0] = cfg.zNear;
+    hp.z_near_far[1] = cfg.zFar;
+}
+
+int ensure_hybrid(avrt_ctx *c, uint32_t W, uint32_t H, size_t cells, uint32_t nlights)
+{
+    HybridState &h = c->hy;
+    int rc = 0;
+    const size_t P = static_cast<size_t>(W) * H;
+    if (h.w != W || h.h != H) {
+        if ((rc = ensure(c, h.gpos, P * 16)) || (rc = ensure(c, h.gnrm, P * 8)) || (rc = ensure(c, h.galb, P * 4)) ||
+            (rc = ensure(c, h.gdep, P * 4)))
+            return rc;
+        for (int i = 0; i < 2; i++) {
+            if ((rc = ensure(c, h.img[i], P * 16)) || (rc = ensure(c, h.geom[i], P * 16))) return rc;
+            HIPX(c, hipMemsetAsync(h.img[i].p, 0, P * 16, c->stream));
+            HIPX(c, hipMemsetAsync(h.geom[i].p, 0, P * 16, c->stream));
+        }
+        h.w = W;
+        h.h = H;
+        h.hist = 0;
+        h.any = false;
+    }
+    if (cells > h.cap_cells) {
+        DevBuf *f4[] = {&h.pr_o, &h.pr_d, &h.pr_hb, &h.lcol, &h.lind, &h.lred, &h.po[0], &h.po[1], &h.pd[0], &h.pd[1],
+                        &h.pt[0], &h.pt[1], &h.nee_tp};
+        for (DevBuf *b : f4)
+            if ((rc = ensure(c, *b, cells * 16))) return rc;
+        if ((rc = ensure(c, h.pr_hit, cells * 16)) || (rc = ensure(c, h.pr_far, cells * 4)) ||
+            (rc = ensure(c, h.rng, cells * 4)) || (rc = ensure(c, h.counters, 64)))
+            return rc;
+        h.cap_cells = cells;
+    }
+    const size_t nq = std::max(cells, cells * std::min<uint32_t>(nlights, kWaveLights));
+    if (nq > h.cap_shadow) {
+        if ((rc = ensure(c, h.so, nq * 16)) || (rc = ensure(c, h.sd, nq * 16)) || (rc = ensure(c, h.sfar, nq * 4)) ||
+            (rc = ensure(c, h.nee_contrib, nq * 16)) || (rc = ensure(c, h.hitA, nq * 16)) ||
+            (rc = ensure(c, h.hitB, nq * 4)))
+            return rc;
+        h.cap_shadow = nq;
+    }
+    return AVRT_OK;
+}
+
+int read_counters4(avrt_ctx *c, uint32_t *out)
+{
+    HIPX(c, hipMemcpyAsync(out, c->hy.counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+// NEE queries of one shade launch: Occluded for each, zero the occluded
+// contributions, then the clamped sums in light order into `target`.
+int run_nee(avrt_ctx *c, const SceneDev &sc, uint32_t nrec, uint32_t nq, uint32_t nl, float4 *target)
+{
+    HybridState &h = c->hy;
+    if (nq) {
+        hipLaunchKernelGGL((k_query<kQueryOccluded>), dim3(blocks_of(nq)), dim3(kBlock), 0, c->stream, sc, nq,
+                           h.so.as<const float4>(), h.sd.as<const float4>(), h.sfar.as<const float>(), h.hitA.as<uint4>(),
+                           h.hitB.as<float>());
+        HIPX(c, hipGetLastError());
+        hipLaunchKernelGGL(k_occl_apply, dim3(blocks_of(nq)), dim3(kBlock), 0, c->stream, nq, h.sd.as<const float4>(),
+                           h.hitA.as<const uint4>(), h.nee_contrib.as<float4>());
+        HIPX(c, hipGetLastError());
+    }
+    if (nrec) {
+        NeeBufs nb{h.nee_tp.as<float4>(), h.nee_contrib.as<float4>()};
+        hipLaunchKernelGGL(k_nee_finalize, dim3(blocks_of(nrec)), dim3(kBlock), 0, c->stream, nrec, nl, nb, target);
+        HIPX(c, hipGetLastError());
+    }
+    c->stats.rays_shadow += nq;
+    return AVRT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+float avrt_halton(uint32_t index, uint32_t base)
+{
+    float result = 0.0f;
+    float fraction = 1.0f;
+    if (base < 2) return 0.0f;
+    while (index > 0) {
+        fraction /= static_cast<float>(base);
+        result += fraction * static_cast<float>(index % base);
+        index /= base;
+    }
+    return result;
+}
+
+int avrt_render_hybrid(avrt_ctx *c, const AvrtConfig *cfg)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    int rc = validate_config(c, cfg);
+    if (rc) return rc;
+    if (c->has_volume) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
+    HIPX(c, hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    if ((rc = ensure_image(c, cfg->width, cfg->height))) return rc;
+    FrameParams fp;
+    frame_params(c, cfg, fp);
+    HybridParams hp;
+    hybrid_params(*cfg, hp);
+    const uint32_t W = cfg->width, H = cfg->height, P = W * H;
+    const uint32_t cells = hp.trace_w * hp.trace_h;
+    if ((rc = ensure_hybrid(c, W, H, cells, c->nlights))) return rc;
+    HybridState &h = c->hy;
+    uchar4 *display = c->d_display.as<uchar4>();
+    HIPX(c, hipEventRecord(c->ev0, c->stream));
+    c->last_hybrid = true;
+    if (cfg->displayOnly != 0) {
+        hipLaunchKernelGGL(k_hy_display_only, dim3(blocks_of(P)), dim3(kBlock), 0, c->stream, fp,
+                           h.img[h.hist].as<const float4>(), display);
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipStreamSynchronize(c->stream));
+        return AVRT_OK;
+    }
+    const SceneDev sc = scene_dev(c);
+    GView g{h.gpos.as<const float4>(), h.gnrm.as<const ushort4>(), h.galb.as<const uchar4>()};
+    hipLaunchKernelGGL(k_gbuffer, dim3(blocks_of(P)), dim3(kBlock), 0, c->stream, sc, fp, hp, h.gpos.as<float4>(),
+                       h.gnrm.as<ushort4>(), h.galb.as<uchar4>(), h.gdep.as<float>());
+    HIPX(c, hipGetLastError());
+    CellState cs{h.pr_o.as<float4>(), h.pr_d.as<float4>(), h.pr_far.as<float>(), h.pr_hit.as<uint4>(),
+                 h.pr_hb.as<float4>(),  h.rng.as<uint32_t>(), h.lcol.as<float4>(),  h.lind.as<float4>()};
+    hipLaunchKernelGGL(k_def_primary, dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, fp, hp, g, cs);
+    HIPX(c, hipGetLastError());
+    hipLaunchKernelGGL((k_query<kQueryNonRaster>), dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, cells,
+                       h.pr_o.as<const float4>(), h.pr_d.as<const float4>(), h.pr_far.as<const float>(), h.hitA.as<uint4>(),
+                       h.hitB.as<float>());
+    HIPX(c, hipGetLastError());
+    hipLaunchKernelGGL(k_def_resolve, dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, fp, hp, g,
+                       h.hitA.as<const uint4>(), h.hitB.as<const float>(), cs);
+    HIPX(c, hipGetLastError());
+    c->stats.rays_primary = cells;
+    c->stats.paths = static_cast<uint64_t>(cells) * hp.spf;
+    if (hp.debug_view != 0) {
+        if (hp.reduced) return set_err(c, AVRT_E_INVALID_ARG, "debug views trace every pixel (traceScale 1)");
+        hipLaunchKernelGGL(k_def_debug, dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, fp, hp, g, cs,
+                           h.gdep.as<const float>(), display);
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipEventRecord(c->ev1, c->stream));
+        HIPX(c, hipEventSynchronize(c->ev1));
+        c->stats.ms_total = elapsed(c->ev0, c->ev1);
+        return AVRT_OK;
+    }
+    const bool wave_nee = c->nlights <= kWaveLights;
+    const uint32_t nl = c->nlights;
+    Queries q{h.so.as<float4>(), h.sd.as<float4>(), h.sfar.as<float>()};
+    NeeBufs rec{h.nee_tp.as<float4>(), h.nee_contrib.as<float4>()};
+    uint32_t *ctr = h.counters.as<uint32_t>();
+    for (uint32_t sample = 0; sample < hp.spf; sample++) {
+        HIPX(c, hipMemsetAsync(ctr, 0, 16, c->stream));
+        PathBufs p0{h.po[0].as<float4>(), h.pd[0].as<float4>(), h.pt[0].as<float4>()};
+        if (wave_nee)
+            hipLaunchKernelGGL((k_def_shade0<true>), dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, fp, hp, g,
+                               cs, sample, p0, q, rec, ctr);
+        else
+            hipLaunchKernelGGL((k_def_shade0<false>), dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, fp, hp,
+                               g, cs, sample, p0, q, rec, ctr);
+        HIPX(c, hipGetLastError());
+        uint32_t cnt[4];
+        if ((rc = read_counters4(c, cnt))) return rc;
+        if (wave_nee && (rc = run_nee(c, sc, cnt[1], cnt[2], nl, h.lcol.as<float4>()))) return rc;
+        uint32_t npaths = cnt[0];
+        int qsel = 0;
+        for (uint32_t bounce = 1; bounce < fp.depth && npaths > 0; bounce++) {
+            c->stats.rays_bounce += npaths;
+            PathBufs in{h.po[qsel].as<float4>(), h.pd[qsel].as<float4>(), h.pt[qsel].as<float4>()};
+            PathBufs out{h.po[qsel ^ 1].as<float4>(), h.pd[qsel ^ 1].as<float4>(), h.pt[qsel ^ 1].as<float4>()};
+            hipLaunchKernelGGL((k_trace_lane<false, false>), dim3(blocks_of(npaths)), dim3(kBlock), 0, c->stream, sc,
+                               npaths, in.o, in.d, static_cast<const float4 *>(nullptr), h.hitA.as<uint4>(),
+                               h.hitB.as<float>());
+            HIPX(c, hipGetLastError());
+            HIPX(c, hipMemsetAsync(ctr, 0, 16, c->stream));
+            if (wave_nee)
+                hipLaunchKernelGGL((k_def_shade<true>), dim3(blocks_of(npaths)), dim3(kBlock), 0, c->stream, sc, fp,
+                                   npaths, bounce, in, h.hitA.as<const uint4>(), h.hitB.as<const float>(), cs, out, q,
+                                   rec, ctr);
+            else
+                hipLaunchKernelGGL((k_def_shade<false>), dim3(blocks_of(npaths)), dim3(kBlock), 0, c->stream, sc, fp,
+                                   npaths, bounce, in, h.hitA.as<const uint4>(), h.hitB.as<const float>(), cs, out, q,
+                                   rec, ctr);
+            HIPX(c, hipGetLastError());
+            if ((rc = read_counters4(c, cnt))) return rc;
+            if (wave_nee && (rc = run_nee(c, sc, cnt[1], cnt[2], nl, h.lind.as<float4>()))) return rc;
+            npaths = cnt[0];
+            qsel ^= 1;
+        }
+        hipLaunchKernelGGL(k_def_sample_end, dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, cells, cs);
+        HIPX(c, hipGetLastError());
+    }
+    const int write = 1 - h.hist, read = h.hist;
+    History hist{h.img[read].as<const float4>(), h.geom[read].as<const float4>(), h.img[write].as<float4>(),
+                 h.geom[write].as<float4>()};
+    hipLaunchKernelGGL(k_def_finish, dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, fp, hp, g, cs, hist,
+                       h.lred.as<float4>(), display);
+    HIPX(c, hipGetLastError());
+    if (hp.reduced) {
+        hipLaunchKernelGGL(k_upsample, dim3(blocks_of(P)), dim3(kBlock), 0, c->stream, sc, fp, hp, g,
+                           h.lred.as<const float4>(), hist, display);
+        HIPX(c, hipGetLastError());
+    }
+    HIPX(c, hipEventRecord(c->ev1, c->stream));
+    HIPX(c, hipEventSynchronize(c->ev1));
+    c->stats.ms_total = elapsed(c->ev0, c->ev1);
+    h.hist = write;
+    h.any = true;
+    return AVRT_OK;
+}
+
+int avrt_read_gbuffer(avrt_ctx *c, float *position, uint16_t *normal_half, uint8_t *albedo, float *depth,
+                      size_t pixels)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    const HybridState &h = c->hy;
+    const size_t P = static_cast<size_t>(h.w) * h.h;
+    if (P == 0) return set_err(c, AVRT_E_STATE, "no hybrid frame rendered yet");
+    if (pixels < P) return set_err(c, AVRT_E_CAPACITY, "destination too small");
+    HIPX(c, hipSetDevice(c->device));
+    if (position) HIPX(c, hipMemcpyAsync(position, h.gpos.p, P * 16, hipMemcpyDeviceToHost, c->stream));
+    if (normal_half) HIPX(c, hipMemcpyAsync(normal_half, h.gnrm.p, P * 8, hipMemcpyDeviceToHost, c->stream));
+    if (albedo) HIPX(c, hipMemcpyAsync(albedo, h.galb.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+    if (depth) HIPX(c, hipMemcpyAsync(depth, h.gdep.p, P * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+int avrt_read_history(avrt_ctx *c, float *image, float *geometry, size_t pixels)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    const HybridState &h = c->hy;
+    const size_t P = static_cast<size_t>(h.w) * h.h;
+    if (P == 0 || !h.any) return set_err(c, AVRT_E_STATE, "no hybrid frame rendered yet");
+    if (pixels < P) return set_err(c, AVRT_E_CAPACITY, "destination too small");
+    HIPX(c, hipSetDevice(c->device));
+    if (image) HIPX(c, hipMemcpyAsync(image, h.img[h.hist].p, P * 16, hipMemcpyDeviceToHost, c->stream));
+    if (geometry) HIPX(c, hipMemcpyAsync(geometry, h.geom[h.hist].p, P * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+int avrt_read_lighting_reduced(avrt_ctx *c, float *dst, size_t cells)
+{
+    if (c == nullptr || dst == nullptr) return AVRT_E_INVALID_ARG;
+    const HybridState &h = c->hy;
+    if (h.cap_cells == 0) return set_err(c, AVRT_E_STATE, "no hybrid frame rendered yet");
+    if (cells > h.cap_cells) return set_err(c, AVRT_E_CAPACITY, "more cells than the context holds");
+    HIPX(c, hipSetDevice(c->device));
+    HIPX(c, hipMemcpyAsync(dst, h.lred.p, cells * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
+} // extern "C"

@@ -15,10 +15,12 @@
 //                 interior = Node4 index; leaf = LEAF | count << 24 | first.
 //                 Each tree starts with a "super-root" whose child 0 is the
 //                 reference root (so the root box is tested, bvh.h:275).
-//                 pad[0] classifies the children's subtrees for shadow
-//                 queries: bit c = child c holds a light or glass object,
-//                 bit 4 + c = child c holds any other object (BLAS nodes:
-//                 all bits set).
+//                 pad[0] classifies the children's subtrees, four bits per
+//                 class, bit c of a class for child c: bits 0-3 hold a light
+//                 or glass object, 4-7 any other object, 8-11 an object the
+//                 rasterizer draws (mesh or raster-proxy box), 12-15 any other
+//                 object (BLAS nodes: all bits set). Filtered walks skip
+//                 children without the class they look for.
 //   TriPre  64 B  per mesh triangle in BLAS slot order: p0, e1, e2 and
 //                 w = n / |n|^2 plus dot(n, p0), exactly the quantities
 //                 HitMeshTriangle (ray_trace.comp.hybrid.hlsl:564-598)
@@ -69,16 +71,20 @@ struct alignas(16) TriPre {
 static_assert(sizeof(TriPre) == 64, "TriPre is 64 bytes");
 
 // ObjGeom word 15 (flags): bits 0-7 shape, bit 8 valid, bit 9 the object is
-// a light (DIFFUSE_LIGHT) or glass (DIELECTRIC): the two materials a shadow
-// ray does not simply stop at.
+// a light (DIFFUSE_LIGHT) or glass (DIELECTRIC), the two materials a shadow
+// ray does not stop at; bit 10 the rasterizer of the hybrid renderer draws
+// it (a mesh, or a box IsRasterProxy accepts, deferred.comp.hlsl:1011-1017).
 constexpr uint32_t kGeomValid = 1u << 8;
 constexpr uint32_t kGeomLG = 1u << 9;
+constexpr uint32_t kGeomRaster = 1u << 10;
 // Subtree classes of the Node4 child mask.
-constexpr uint32_t kClassLG = 1u, kClassOther = 2u;
+constexpr uint32_t kClassLG = 1u, kClassOther = 2u, kClassRaster = 4u, kClassNonRaster = 8u;
 inline constexpr uint32_t child_mask_bits(int c, uint32_t cls)
 {
-    return ((cls & kClassLG) ? (1u << c) : 0u) | ((cls & kClassOther) ? (1u << (4 + c)) : 0u);
+    return ((cls & kClassLG) ? (1u << c) : 0u) | ((cls & kClassOther) ? (1u << (4 + c)) : 0u) |
+           ((cls & kClassRaster) ? (1u << (8 + c)) : 0u) | ((cls & kClassNonRaster) ? (1u << (12 + c)) : 0u);
 }
+constexpr uint32_t kClassAll = kClassLG | kClassOther | kClassRaster | kClassNonRaster;
 
 struct alignas(16) ObjGeom {
     float a[4];
@@ -105,6 +111,25 @@ struct FrameParams {
     uint32_t fp16;
     uint32_t has_texture;
     uint32_t tex_w, tex_h, tex_layers;
+};
+
+// Per-frame constants of the hybrid frame (deferred.comp / upsample.comp
+// uniforms beyond FrameParams), computed on the host from AvrtConfig.
+struct HybridParams {
+    float jitter_x, jitter_y;  // sub-pixel offset of this frame's primary rays
+    float trace_scale;         // TraceScale
+    uint32_t frame_index, history_reset, camera_moved;
+    float history_cap;
+    float tan_half_fov;        // tan(radians(Fov / 2)), the camera's own value
+    uint32_t trace_w, trace_h; // TraceExtent (== width, height at full scale)
+    uint32_t reduced;          // trace_scale < 1
+    uint32_t spf;              // samples this frame: max(Samples / Batches, 1)
+    uint32_t debug_view;
+    float prev_view_proj[16];  // row-major, last frame's unjittered ViewProj
+    float view_w[3];           // camera backward axis w (LookAtRH row 2)
+    float view_wd;             // dot(w, eye)
+    float proj_m22, proj_m23;  // PerspectiveRH_ZO depth row
+    float z_near_far[2];       // ZNear, ZFar (LinearizeDepth, mat4.h:100-103)
 };
 
 } // namespace avrt

@@ -614,16 +614,20 @@ __device__ ObjGeom make_geom(const AvrtObject &o, const AvrtMeshInstance *inst, 
         if (m.triangleCount > 0) flags |= kGeomValid;
     }
     if (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC) flags |= kGeomLG;
+    if (o.shapeType == AVRT_SHAPE_MESH) flags |= kGeomRaster;
     g.d[3] = flags;
     return g;
 }
 
-// Also writes lg_prefix[s] = number of light or glass objects in slots
-// [0, s), for s = 0..n (the child masks of k_convert_tlas).
+// Also writes lg_prefix[s] and raster_prefix[s], the numbers of light or
+// glass objects and of rasterized objects (meshes: the device rebuild takes
+// no boxes) in slots [0, s), for s = 0..n (the child masks of
+// k_convert_tlas).
 __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order, const AvrtObject *auth,
                                               const int32_t *mesh_rank, const AvrtMeshInstance *auth_inst,
                                               const uint4 *geomc, AvrtObject *objects, AvrtMeshInstance *instances,
-                                              ObjGeom *geom, uint32_t *lights, uint32_t *lg_prefix)
+                                              ObjGeom *geom, uint32_t *lights, uint32_t *lg_prefix,
+                                              uint32_t *raster_prefix)
 {
     __shared__ uint32_t sh_wm[kBT / 64], sh_wl[kBT / 64], sh_wg[kBT / 64];
     __shared__ uint32_t sh_cm, sh_cl, sh_cg;
@@ -683,7 +687,11 @@ __global__ __launch_bounds__(kBT) void k_pack(uint32_t n, const uint32_t *order,
             geom[s] = make_geom(o, auth_inst, geomc, rank);
             if (il) lights[rl] = s;
             lg_prefix[s] = rg;
-            if (s == n - 1) lg_prefix[n] = rg + ig;
+            raster_prefix[s] = rm;
+            if (s == n - 1) {
+                lg_prefix[n] = rg + ig;
+                raster_prefix[n] = rm + im;
+            }
         }
         __syncthreads();
         if (tid == kBT - 1) {
@@ -719,11 +727,14 @@ __device__ __forceinline__ uint32_t ref_of(const AvrtBVHNode *nodes, uint32_t j,
     return 1u + j;
 }
 
-__device__ __forceinline__ uint32_t class_of(const uint2 *range, const uint32_t *lg_prefix, uint32_t j)
+__device__ __forceinline__ uint32_t class_of(const uint2 *range, const uint32_t *lg_prefix,
+                                             const uint32_t *raster_prefix, uint32_t j)
 {
     const uint2 r = range[j];
     const uint32_t lg = lg_prefix[r.y] - lg_prefix[r.x];
-    return (lg ? kClassLG : 0u) | ((r.y - r.x) > lg ? kClassOther : 0u);
+    const uint32_t ras = raster_prefix[r.y] - raster_prefix[r.x];
+    return (lg ? kClassLG : 0u) | ((r.y - r.x) > lg ? kClassOther : 0u) | (ras ? kClassRaster : 0u) |
+           ((r.y - r.x) > ras ? kClassNonRaster : 0u);
 }
 
 // Node4 index of an even-depth interior node j is 1 + j (odd-depth slots stay
@@ -731,7 +742,8 @@ __device__ __forceinline__ uint32_t class_of(const uint2 *range, const uint32_t 
 // converter: each interior child contributes its two children. Child masks
 // come from the slot range of each child's subtree.
 __global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes, const uint32_t *depth,
-                               const uint2 *range, const uint32_t *lg_prefix, Node4 *out, uint32_t *info)
+                               const uint2 *range, const uint32_t *lg_prefix, const uint32_t *raster_prefix,
+                               Node4 *out, uint32_t *info)
 {
     const uint32_t nn = info_in[0];
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,7 +755,7 @@ __global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes
     uint32_t mask = 0;
     if (j == 0) {
         set_child(n, 0, nodes[0], ref_of(nodes, 0, info));
-        mask = child_mask_bits(0, class_of(range, lg_prefix, 0));
+        mask = child_mask_bits(0, class_of(range, lg_prefix, raster_prefix, 0));
     } else {
         const uint32_t me = j - 1;
         const AvrtBVHNode &b = nodes[me];
@@ -756,12 +768,12 @@ __global__ void k_convert_tlas(const uint32_t *info_in, const AvrtBVHNode *nodes
                     for (int g = 0; g < 2; g++) {
                         const uint32_t q = static_cast<uint32_t>(g == 0 ? ch.left : ch.right);
                         set_child(n, c, nodes[q], ref_of(nodes, q, info));
-                        mask |= child_mask_bits(c, class_of(range, lg_prefix, q));
+                        mask |= child_mask_bits(c, class_of(range, lg_prefix, raster_prefix, q));
                         c++;
                     }
                 } else {
                     set_child(n, c, ch, ref_of(nodes, k, info));
-                    mask |= child_mask_bits(c, class_of(range, lg_prefix, k));
+                    mask |= child_mask_bits(c, class_of(range, lg_prefix, raster_prefix, k));
                     c++;
                 }
             }
@@ -854,11 +866,12 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
                            rb.auth_objects.as<AvrtObject>(), rb.mesh_rank.as<int32_t>(),
                            rb.auth_inst.as<AvrtMeshInstance>(), rb.inst_geomc.as<uint4>(), c->d_objects.as<AvrtObject>(),
                            c->d_instances.as<AvrtMeshInstance>(), c->d_geom.as<ObjGeom>(), c->d_lights.as<uint32_t>(),
-                           rb.lg_prefix.as<uint32_t>());
+                           rb.lg_prefix.as<uint32_t>(), rb.raster_prefix.as<uint32_t>());
         HIPX(c, hipGetLastError());
         hipLaunchKernelGGL(k_convert_tlas, dim3(blocks(2 * n + 1, 256)), dim3(256), 0, c->stream, rb.info.as<uint32_t>(),
                            rb.nodes.as<AvrtBVHNode>(), rb.depth.as<uint32_t>(), rb.range.as<uint2>(),
-                           rb.lg_prefix.as<uint32_t>(), c->d_tlas2.as<Node4>(), rb.info.as<uint32_t>());
+                           rb.lg_prefix.as<uint32_t>(), rb.raster_prefix.as<uint32_t>(), c->d_tlas2.as<Node4>(),
+                           rb.info.as<uint32_t>());
         HIPX(c, hipGetLastError());
     }
     HIPX(c, hipEventRecord(c->ev_b, c->stream));
@@ -946,6 +959,7 @@ int avrt_scene_set_authoring(avrt_ctx *c, const AvrtObject *objects_in, uint32_t
     if ((rc = ensure(c, rb.depth, sizeof(uint32_t) * 2 * (n ? n : 1)))) return rc;
     if ((rc = ensure(c, rb.range, sizeof(uint2) * 2 * (n ? n : 1)))) return rc;
     if ((rc = ensure(c, rb.lg_prefix, sizeof(uint32_t) * (n + 1)))) return rc;
+    if ((rc = ensure(c, rb.raster_prefix, sizeof(uint32_t) * (n + 1)))) return rc;
     rb.n = object_count;
     rb.m = instance_count;
     rb.nlights = nl;

@@ -1,0 +1,471 @@
+// avrt_shade.h - device code shared by the frame kernels of libavrt: queue
+// compaction, texture sampling, hit attributes, bounce sampling, the
+// traversal kernels, NEE accumulation and display encoding.
+//
+// Included by each HIP translation unit of the library; everything here has
+// internal linkage (one copy per unit, including the sRGB table in constant
+// memory, which each unit uploads for itself).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "../../include/avrt.h"
+#include "avrt_device.h"
+#include "avrt_internal.h"
+
+namespace {
+
+using namespace avrt;
+using namespace avrt::dev;
+
+constexpr int kBlock = 256;
+constexpr int kStackLds = 16;
+constexpr int kStackSpill = 96;
+constexpr int kStepsPerRefill = 24;   // traversal steps between lane refills
+constexpr uint32_t kWaveLights = 4;   // NEE as shadow-ray wavefronts up to this many lights
+
+// sRGB-encoded byte -> linear, uploaded per translation unit.
+__constant__ float c_srgb[256];
+
+// ---------------------------------------------------------------------------
+// Wave-level queue compaction: returns this lane's output slot (valid only if
+// pred) with one atomic add per wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t *counter)
+{
+    const uint64_t active = __ballot(1);
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane_off =
+        __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    const int leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(__lane_id()) == leader && mask != 0ull) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + lane_off;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Texture: bilinear, REPEAT, level 0, per-texel sRGB decode.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+__device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t layer, float u, float v)
+{
+    if (fp.has_texture == 0) return f3(1.0f, 1.0f, 1.0f);
+    if (layer >= fp.tex_layers) layer = fp.tex_layers - 1;
+    if (!isfinite(u)) u = 0.0f;
+    if (!isfinite(v)) v = 0.0f;
+    const float x = u * static_cast<float>(fp.tex_w) - 0.5f;
+    const float y = v * static_cast<float>(fp.tex_h) - 0.5f;
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int W = static_cast<int>(fp.tex_w), H = static_cast<int>(fp.tex_h);
+    const int i0 = wrapi(static_cast<int>(fmodf(x0, static_cast<float>(W))), W);
+    const int j0 = wrapi(static_cast<int>(fmodf(y0, static_cast<float>(H))), H);
+    const int i1 = (i0 + 1) % W, j1 = (j0 + 1) % H;
+    const uchar4 *base = sc.tex + static_cast<size_t>(layer) * W * H;
+    const uchar4 a = base[static_cast<size_t>(j0) * W + i0];
+    const uchar4 b = base[static_cast<size_t>(j0) * W + i1];
+    const uchar4 c = base[static_cast<size_t>(j1) * W + i0];
+    const uchar4 e = base[static_cast<size_t>(j1) * W + i1];
+    const F3 t00 = f3(c_srgb[a.x], c_srgb[a.y], c_srgb[a.z]);
+    const F3 t10 = f3(c_srgb[b.x], c_srgb[b.y], c_srgb[b.z]);
+    const F3 t01 = f3(c_srgb[c.x], c_srgb[c.y], c_srgb[c.z]);
+    const F3 t11 = f3(c_srgb[e.x], c_srgb[e.y], c_srgb[e.z]);
+    const F3 top = t00 + fx * (t10 - t00);
+    const F3 bot = t01 + fx * (t11 - t01);
+    return top + fy * (bot - top);
+}
+
+// ---------------------------------------------------------------------------
+// Hit attributes (position, facing normal, surface uv) for a closest hit.
+// ---------------------------------------------------------------------------
+struct Surf {
+    F3 pos, normal;
+    float u, v;
+    bool face;
+};
+
+__device__ void hit_attributes(const SceneDev &sc, const AvrtObject &ob, const HitRec &h, F3 o, F3 d, float time,
+                               Surf &s)
+{
+    s.u = 0.0f;
+    s.v = 0.0f;
+    const uint32_t shape = ob.shapeType;
+    if (shape == AVRT_SHAPE_SPHERE) {
+        const F3 P = f3(ob.x, ob.y, ob.z), P2 = f3(ob.x2, ob.y2, ob.z2);
+        const F3 centre = lerp(P, P2, time);
+        s.pos = o + d * h.t;
+        s.normal = (s.pos - centre) / ob.radius;
+        s.face = dot(d, s.normal) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        if (ob.textureID != AVRT_NO_TEXTURE) {
+            const float theta = acosf(-s.normal.y);
+            const float phi = atan2f(-s.normal.z, s.normal.x) + 3.14159265f;
+            const float a = phi / 6.2831853f - ob.rotation / 6.2831853f;
+            s.u = a - floorf(a);
+            s.v = theta / 3.14159265f;
+        }
+    } else if (shape == AVRT_SHAPE_QUAD || shape == AVRT_SHAPE_TRIANGLE) {
+        const F3 n = cross(f3(ob.u_x, ob.u_y, ob.u_z), f3(ob.v_x, ob.v_y, ob.v_z));
+        const float nl = dot(n, n);
+        s.pos = o + d * h.t;
+        s.normal = n / sqrtf(nl);
+        s.face = dot(d, s.normal) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        s.u = h.alpha;
+        s.v = h.beta;
+    } else if (shape == AVRT_SHAPE_MESH) {
+        const ObjGeom &g = sc.geom[h.obj];
+        const uint32_t tri_base = __float_as_uint(g.c[1]);
+        const uint32_t vbase = __float_as_uint(g.c[2]);
+        const uint32_t ibase = __float_as_uint(g.c[3]);
+        const uint32_t base = ibase + h.prim * 3u;
+        const AvrtVertex a = sc.verts[vbase + sc.indices[base + 0]];
+        const AvrtVertex b = sc.verts[vbase + sc.indices[base + 1]];
+        const AvrtVertex c = sc.verts[vbase + sc.indices[base + 2]];
+        const TriPre &tp = sc.tris[tri_base + h.prim];
+        const F3 n = cross(ld3(tp.e1), ld3(tp.e2));
+        F3 geo = n / sqrtf(dot(n, n));
+        const float w0 = 1.0f - h.alpha - h.beta;
+        const F3 shl = normalize(w0 * f3(a.nx, a.ny, a.nz) + h.alpha * f3(b.nx, b.ny, b.nz) + h.beta * f3(c.nx, c.ny, c.nz));
+        if (dot(geo, shl) < 0.0f) geo = -geo;
+        const float qx = g.b[0], qy = g.b[1], qz = g.b[2], qw = g.b[3];
+        s.pos = o + d * h.t;
+        s.normal = normalize(quat_rotate(qx, qy, qz, qw, shl));
+        s.face = dot(d, quat_rotate(qx, qy, qz, qw, geo)) < 0.0f;
+        if (!s.face) s.normal = -s.normal;
+        s.u = w0 * a.u + h.alpha * b.u + h.beta * c.u;
+        s.v = w0 * a.v + h.alpha * b.v + h.beta * c.v;
+    } else {
+        const ObjGeom &g = sc.geom[h.obj];
+        BoxLocal r;
+        box_local(g, o, d, r);
+        const bool outside = (h.t == r.tin);
+        const F3 ln = outside ? r.en : r.ex;
+        s.pos = o + d * h.t;
+        const F3 lp = r.lo + r.ld * h.t;
+        const F3 half = f3(g.b[0], g.b[1], g.b[2]);
+        float uu, vv;
+        if (fabsf(ln.x) > 0.5f) {
+            uu = lp.z / half.z;
+            vv = lp.y / half.y;
+        } else if (fabsf(ln.y) > 0.5f) {
+            uu = lp.x / half.x;
+            vv = lp.z / half.z;
+        } else {
+            uu = lp.x / half.x;
+            vv = lp.y / half.y;
+        }
+        s.u = saturate(uu * 0.5f + 0.5f);
+        s.v = saturate(vv * 0.5f + 0.5f);
+        s.normal = rotate_y(ln, g.a[3], g.b[3]);
+        s.face = outside;
+        if (!s.face) s.normal = -s.normal;
+    }
+}
+
+// GetBounce direction (shared by both shaders).
+__device__ __forceinline__ bool get_bounce(const AvrtObject &ob, F3 d, const Surf &s, Rng &rng, F3 *bounce)
+{
+    *bounce = f3(0.0f, 0.0f, 0.0f);
+    switch (ob.colorType) {
+    case AVRT_LAMBERTIAN: {
+        const F3 rd = rng.direction();
+        *bounce = s.normal + rd;
+        if (dot(*bounce, *bounce) < 0.001f) *bounce = s.normal;
+        return true;
+    }
+    case AVRT_METAL: {
+        F3 b = normalize(reflect(d, s.normal));
+        const F3 rd = rng.direction();
+        b = b + rd * ob.fuzz;
+        *bounce = b;
+        return dot(b, s.normal) > 0.0f;
+    }
+    case AVRT_DIELECTRIC: {
+        const float ior = ob.refraction;
+        const float ri = s.face ? (1.0f / ior) : ior;
+        const F3 dir = normalize(d);
+        const float cosT = fminf(dot(-dir, s.normal), 1.0f);
+        const float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+        const float q = (1.0f - ri) / (1.0f + ri);
+        const float r0 = q * q;
+        const float m = 1.0f - cosT;
+        const float m5 = ((m * m) * (m * m)) * m;
+        const float refl = r0 + (1.0f - r0) * m5;
+        if (ri * sinT > 1.0f || refl > rng.next()) *bounce = reflect(dir, s.normal);
+        else *bounce = refract(dir, s.normal, ri);
+        return true;
+    }
+    case AVRT_ISOTROPIC:
+        *bounce = rng.direction();
+        return true;
+    default:
+        return false;
+    }
+}
+
+// One light of CalculateDirectLight (ray_trace.comp.hybrid.hlsl:1131-1149,
+// deferred.comp.hlsl:1385-1403, identical in both): SampleLight (:1057-1119)
+// and the early outs. On true, *lp is the sampled point, *dir the unit
+// direction from `pos`, *dist the distance and *contrib the unoccluded
+// contribution brdf * radiance * cos * solid angle.
+__device__ __forceinline__ bool nee_light(const SceneDev &sc, uint32_t slot, F3 pos, F3 normal, F3 brdf, Rng &rng,
+                                          F3 *lp_out, F3 *dir_out, float *dist_out, F3 *contrib)
+{
+    const AvrtObject l = sc.objects[slot];
+    const F3 LP = f3(l.x, l.y, l.z);
+    F3 lp = LP;
+    float solid = 0.0f;
+    if (l.shapeType == AVRT_SHAPE_QUAD) {
+        const F3 lu = f3(l.u_x, l.u_y, l.u_z), lv = f3(l.v_x, l.v_y, l.v_z);
+        const float r1 = rng.next();
+        const float r2 = rng.next();
+        lp = LP + r1 * lu + r2 * lv;
+        const F3 cuv = cross(lu, lv);
+        const float area = length(cuv);
+        const F3 lnrm = cuv / fmaxf(area, 1e-8f);
+        F3 ts = pos - lp;
+        const float d2 = dot(ts, ts);
+        ts = normalize(ts);
+        const float cl = fabsf(dot(lnrm, ts));
+        solid = (cl * area) / fmaxf(d2, 1e-8f);
+    } else if (l.shapeType == AVRT_SHAPE_SPHERE) {
+        const F3 tl = LP - pos;
+        const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+        const float r2 = l.radius * l.radius;
+        if (d2 <= r2) {
+            solid = 4.0f * 3.14159265f;
+        } else {
+            const float ctm = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+            solid = 2.0f * 3.14159265f * (1.0f - ctm);
+        }
+    } else {
+        const F3 tl = LP - pos;
+        const float d2 = fmaxf(dot(tl, tl), 1e-8f);
+        const float area = 4.0f * (l.half_x * l.half_y + l.half_y * l.half_z + l.half_z * l.half_x) / 6.0f;
+        solid = area / d2;
+    }
+    if (solid <= 0.0f) return false;
+    F3 dir = lp - pos;
+    const float dist = length(dir);
+    if (dist <= 1e-6f) return false;
+    dir = dir / dist;
+    const float ct = dot(normal, dir);
+    if (ct <= 0.0f) return false;
+    const F3 radiance = l.emission * f3(l.r, l.g, l.b);
+    *contrib = brdf * radiance * ct * solid;
+    *lp_out = lp;
+    *dir_out = dir;
+    *dist_out = dist;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+struct PathBufs {
+    float4 *o;  // origin.xyz, time
+    float4 *d;  // direction.xyz, path id (bits)
+    float4 *t;  // throughput.xyz, rng state (bits)
+};
+
+struct ShadowBufs {
+    float4 *p;  // surface position.xyz, time
+    float4 *n;  // facing normal.xyz, path id
+    float4 *a;  // surface albedo.xyz, rng state at NEE
+    float4 *w;  // throughput.xyz
+};
+
+
+// Closest-hit traversal of a ray queue, one ray per lane. SHADOW rays are
+// bounded by the light point stored beside them: far = |lp - o| - 0.001
+// (CalculateDirectLight, ray_trace.comp.hybrid.hlsl:1167-1179).
+template <bool REF_ORDER, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace_lane(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                       const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                       uint4 *__restrict__ hitA, float *__restrict__ hitB)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+    uint32_t spill_mem[kStackSpill];
+    const float4 O = ro[idx];
+    const float4 D = rd[idx];
+    const F3 o = f3(O.x, O.y, O.z);
+    float far = 1000000.0f;
+    if (SHADOW) {
+        const float4 L = rlp[idx];
+        far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
+    }
+    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
+    if (tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem)) {
+        while (tr.step(sc)) {
+        }
+    }
+    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+    hitB[idx] = tr.hit.beta;
+}
+
+// Queries of the hybrid frame, one ray per lane, each bounded by its own far
+// distance rfar[idx] (near 0.001, children nearest first like
+// TraverseBVHBounded, deferred.comp.hlsl:1027-1099):
+//   kQueryRaster     closest hit among what the rasterizer draws (meshes and
+//                    raster-proxy boxes): the ray-cast G-buffer;
+//   kQueryNonRaster  closest hit among everything else: ResolvePrimary's
+//                    analytic trace (skipRasterized, :1526-1548);
+//   kQueryOccluded   any hit of an object that is neither glass nor a light
+//                    (Occluded / OccludedMesh, :1115-1171, 812-854).
+constexpr int kQueryRaster = 0, kQueryNonRaster = 1, kQueryOccluded = 2;
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_query(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                  const float4 *__restrict__ rd, const float *__restrict__ rfar,
+                                                  uint4 *__restrict__ hitA, float *__restrict__ hitB)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+    uint32_t spill_mem[kStackSpill];
+    const float4 O = ro[idx];
+    const float4 D = rd[idx];
+    constexpr int F = KIND == kQueryRaster ? kRaster : (KIND == kQueryNonRaster ? kNonRaster : kOpaqueAny);
+    Traversal<false, kStackLds, kBlock, kStackSpill, F> tr;
+    if (tr.init(sc, f3(O.x, O.y, O.z), f3(D.x, D.y, D.z), O.w, 0.001f, rfar[idx], lds_stack, spill_mem)) {
+        if (KIND == kQueryOccluded) tr.term = INFINITY;
+        while (tr.step(sc)) {
+        }
+    }
+    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+    hitB[idx] = tr.hit.beta;
+}
+
+// Closest-hit traversal of a ray queue by persistent lanes: a lane whose ray
+// is finished takes the next unclaimed ray (one atomic per wave), so a wave
+// does not idle behind its slowest ray. SHADOW rays are bounded by the light
+// point stored beside them: far = |lp - o| - 0.001 (CalculateDirectLight,
+// ray_trace.comp.hybrid.hlsl:1167-1179).
+template <bool REF_ORDER, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                  const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                  uint4 *__restrict__ hitA, float *__restrict__ hitB, uint32_t *work)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    uint32_t spill_mem[kStackSpill];
+    Traversal<REF_ORDER, kStackLds, kBlock, kStackSpill> tr;
+    uint32_t idx = 0;
+    bool have = false, done = false;
+    while (true) {
+        const bool need = !have && !done;
+        const uint64_t m = __ballot(need);
+        if (m != 0ull) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+            const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+            uint32_t base = 0;
+            if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(work, static_cast<uint32_t>(__popcll(m)));
+            base = __shfl(base, leader);
+            if (need) {
+                idx = base + rank;
+                if (idx < n) {
+                    const float4 O = ro[idx];
+                    const float4 D = rd[idx];
+                    const F3 o = f3(O.x, O.y, O.z);
+                    float far = 1000000.0f;
+                    if (SHADOW) {
+                        const float4 L = rlp[idx];
+                        far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
+                    }
+                    have = tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem);
+                    if (!have) {
+                        hitA[idx] = make_uint4(__float_as_uint(far), ~0u, ~0u, 0u);
+                        hitB[idx] = 0.0f;
+                    }
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0ull) {
+            if (__ballot(!done) == 0ull) break;
+            continue;
+        }
+        if (have) {
+            for (int it = 0; it < kStepsPerRefill; it++) {
+                if (!tr.step(sc)) {
+                    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+                    hitB[idx] = tr.hit.beta;
+                    have = false;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)
+{
+    float4 L = Lpath[pid];
+    L.x = L.x + c.x;
+    L.y = L.y + c.y;
+    L.z = L.z + c.z;
+    Lpath[pid] = L;
+}
+
+// MODE 0 = path_trace.comp, MODE 1 = ray_trace.comp.hybrid.
+struct ShadowRays {
+    float4 *o;   // origin.xyz, time
+    float4 *d;   // direction.xyz, contribution slot (bits)
+    float4 *lp;  // sampled light point.xyz, glass steps taken (bits)
+};
+
+struct NeeBufs {
+    float4 *tp;      // per record: throughput at the hit.xyz, path id
+    float4 *contrib; // per (record, light): unoccluded contribution
+};
+
+
+// Sum of the light contributions in light order, clamped to 8 and weighted by
+// the throughput (ColorRay, ray_trace.comp.hybrid.hlsl:1264-1274).
+__global__ __launch_bounds__(kBlock) void k_nee_finalize(uint32_t n, uint32_t nl, NeeBufs nee, float4 *Lpath)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    F3 sum = f3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < nl; i++) {
+        const float4 c = nee.contrib[r * nl + i];
+        sum = sum + f3(c.x, c.y, c.z);
+    }
+    const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
+    if (luma > 8.0f) sum = sum * (8.0f / luma);
+    const float4 T = nee.tp[r];
+    add_radiance(Lpath, __float_as_uint(T.w), f3(T.x, T.y, T.z) * sum);
+}
+
+
+__device__ __forceinline__ float aces(float x)
+{
+    return saturate((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f));
+}
+__device__ __forceinline__ float to_srgb(float c)
+{
+    c = saturate(c);
+    if (c <= 0.0031308f) return c * 12.92f;
+    return 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ unsigned char unorm8(float x) { return static_cast<unsigned char>(floorf(saturate(x) * 255.0f + 0.5f)); }
+__device__ __forceinline__ float round_half(float x) { return __half2float(__float2half(x)); }
+
+} // namespace

@@ -50,7 +50,7 @@ struct Converter {
     }
     uint32_t class_of_range(uint32_t first, uint32_t count)
     {
-        if (slot_class == nullptr) return kClassLG | kClassOther;
+        if (slot_class == nullptr) return kClassAll;
         uint32_t cls = 0;
         for (uint32_t i = first; i < first + count && i < nslots; i++) cls |= slot_class[i];
         return cls;
@@ -176,9 +176,18 @@ struct Converter {
 
 } // namespace
 
+bool is_rasterized(const AvrtObject &o)
+{
+    if (o.shapeType == AVRT_SHAPE_MESH) return true;
+    return o.shapeType == AVRT_SHAPE_BOX && o.colorType != AVRT_ISOTROPIC && o.textureID == AVRT_NO_TEXTURE &&
+           o.rotation == 0.0f;
+}
+
 uint8_t object_class(const AvrtObject &o)
 {
-    return (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC) ? kClassLG : kClassOther;
+    const uint32_t lg = (o.colorType == AVRT_DIFFUSE_LIGHT || o.colorType == AVRT_DIELECTRIC) ? kClassLG : kClassOther;
+    const uint32_t ras = is_rasterized(o) ? kClassRaster : kClassNonRaster;
+    return static_cast<uint8_t>(lg | ras);
 }
 
 bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
@@ -275,7 +284,8 @@ int build_geom_records(const AvrtObject *objects, uint32_t n, const AvrtMeshInst
             break;
         }
         }
-        if (object_class(o) == kClassLG) flags |= kGeomLG;
+        if (object_class(o) & kClassLG) flags |= kGeomLG;
+        if (is_rasterized(o)) flags |= kGeomRaster;
         g.d[3] = flags;
     }
     return AVRT_OK;

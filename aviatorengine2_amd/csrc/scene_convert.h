@@ -21,7 +21,12 @@ namespace avrt {
 bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
                   std::string *err, const uint8_t *slot_class = nullptr, uint32_t nslots = 0);
 
-// kClassLG for DIFFUSE_LIGHT and DIELECTRIC objects, else kClassOther.
+// IsRasterProxy or a mesh: what the hybrid renderer rasterizes
+// (deferred.comp.hlsl:1011-1017, renderer.h IsRasterProxy).
+bool is_rasterized(const AvrtObject &o);
+
+// kClassLG for DIFFUSE_LIGHT and DIELECTRIC objects, else kClassOther; plus
+// kClassRaster or kClassNonRaster.
 uint8_t object_class(const AvrtObject &o);
 
 // Triangle record of HitMeshTriangle's derived quantities.

@@ -99,6 +99,40 @@ class Renderer:
         self._check(self.lib.avrt_render(self.ctx, c.ctypes.data_as(ctypes.c_void_p), int(mode)), "avrt_render")
         return self.Stats()
 
+    def RenderHybrid(self, config):
+        """One hybrid frame (avrt_render_hybrid): the config's hybrid uniforms
+        as Renderer::RenderFrame sets them (see hybrid.HybridFrames)."""
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        self.width, self.height = int(c["width"][0]), int(c["height"][0])
+        self._check(self.lib.avrt_render_hybrid(self.ctx, c.ctypes.data_as(ctypes.c_void_p)), "avrt_render_hybrid")
+        return self.Stats()
+
+    def ReadGBuffer(self):
+        """(position+slot float32 [H,W,4], normal+coverage float16 [H,W,4],
+        albedo+flag uint8 [H,W,4], depth float32 [H,W])."""
+        H, W = self.height, self.width
+        pos = np.empty((H, W, 4), np.float32)
+        nrm = np.empty((H, W, 4), np.float16)
+        alb = np.empty((H, W, 4), np.uint8)
+        dep = np.empty((H, W), np.float32)
+        self._check(self.lib.avrt_read_gbuffer(self.ctx, ptr(pos), ptr(nrm), ptr(alb), ptr(dep), H * W),
+                    "avrt_read_gbuffer")
+        return pos, nrm, alb, dep
+
+    def ReadHistory(self):
+        """(accumulation [H,W,4], geometry [H,W,4]) of the last traced hybrid frame."""
+        H, W = self.height, self.width
+        img = np.empty((H, W, 4), np.float32)
+        geo = np.empty((H, W, 4), np.float32)
+        self._check(self.lib.avrt_read_history(self.ctx, ptr(img), ptr(geo), H * W), "avrt_read_history")
+        return img, geo
+
+    def ReadLightingReduced(self, cells_w, cells_h):
+        out = np.empty((cells_h, cells_w, 4), np.float32)
+        self._check(self.lib.avrt_read_lighting_reduced(self.ctx, ptr(out), cells_w * cells_h),
+                    "avrt_read_lighting_reduced")
+        return out
+
     def Stats(self) -> dict:
         s = AvrtStats()
         self._check(self.lib.avrt_get_stats(self.ctx, ctypes.byref(s)), "avrt_get_stats")

@@ -35,7 +35,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import abi
-from .abi import (AABB_DTYPE, CONFIG_DTYPE, DIELECTRIC, DIFFUSE_LIGHT, INSTANCE_DTYPE, LAMBERTIAN, MESH, METAL,
+from .abi import (AABB_DTYPE, BOX, CONFIG_DTYPE, DIELECTRIC, DIFFUSE_LIGHT, INSTANCE_DTYPE, LAMBERTIAN, MESH, METAL,
                   NODE_DTYPE, NO_TEXTURE, OBJECT_DTYPE, QUAD, SPHERE, TRIANGLE, VERTEX_DTYPE)
 
 M32 = 0xFFFFFFFF
@@ -415,6 +415,48 @@ def icosphere_tris(seed: int = 0x1C05F3E6, width=1280, height=720, spp=4, depth=
     return SceneSpec("icosphere_tris", abi.MODE_RAY_TRACE, objs, config=cfg, seed=seed)
 
 
+def hybrid_scene(seed: int = 0x1C05F3E6, width=1920, height=1080, spp=1, depth=5, level=6, tex_size=512,
+                 extras=True) -> SceneSpec:
+    """The C2 scene in its two-level form, for the hybrid renderer: the
+    noise-displaced level-6 icosphere (81,920 triangles, Perlin-textured) and
+    the two-triangle ground as mesh instances - what the reference rasterizes
+    into the G-buffer - and the emissive quad light (analytic, radiance 15).
+    With `extras`, a glass and a metal sphere (analytic, resolved against the
+    G-buffer depth) and a plain unrotated box (a raster proxy) join them, so
+    every primary source and both demodulation cases occur."""
+    rng = Pcg32(seed)
+    V, F = icosphere(level)
+    disp = 1.0 + 0.1 * fbm3(V, seed & 0xFFFF)
+    P = (V * disp[:, None]).astype(np.float32)
+    alb_sphere = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+    alb_ground = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+    sphere = Mesh(P, _vertex_normals(P.astype(np.float64), F).astype(np.float32), sphere_uv(V).astype(np.float32),
+                  F.astype(np.uint32))
+    g = np.array([[-6, -1.15, -6], [6, -1.15, -6], [6, -1.15, 6], [-6, -1.15, 6]], np.float32)
+    ground = Mesh(g, np.tile(np.array([[0, 1, 0]], np.float32), (4, 1)),
+                  np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32),
+                  np.array([[0, 2, 1], [0, 3, 2]], np.uint32))
+    objs = [_obj(QUAD, DIFFUSE_LIGHT, (-1.0, 3.5, -1.0), u=(2.0, 0.0, 0.0), v=(0.0, 0.0, 2.0), emission=15.0),
+            _obj(MESH, LAMBERTIAN, (0.0, 0.0, 0.0), albedo=alb_sphere, texture=0 if tex_size else NO_TEXTURE),
+            _obj(MESH, LAMBERTIAN, (0.0, 0.0, 0.0), albedo=alb_ground)]
+    if extras:
+        objs.append(_obj(SPHERE, DIELECTRIC, (1.55, -0.6, 1.3), 0.55, ior=1.5))
+        objs.append(_obj(SPHERE, METAL, (-1.7, -0.62, 1.0), 0.53, albedo=(0.85, 0.8, 0.7), fuzz=0.05))
+        box = _obj(BOX, LAMBERTIAN, (0.15, -0.85, 2.1), albedo=(0.7, 0.3, 0.25))
+        box["half_x"], box["half_y"], box["half_z"] = 0.3, 0.3, 0.3
+        objs.append(box)
+    inst = np.zeros(2, INSTANCE_DTYPE)
+    inst["qw"] = 1.0
+    inst["scale"] = 1.0
+    tex = perlin_texture(tex_size, seed & 0xFFFF)[None] if tex_size else None
+    src = (0.0, 1.2, 4.5)
+    focus = math.sqrt(sum(x * x for x in src))
+    cfg = make_config(width, height, spp, depth, src, (0, 0, 0), 40.0, focus, 0.0, (0.02, 0.02, 0.03), exposure=1.0)
+    return SceneSpec("hybrid_c2", abi.MODE_RAY_TRACE, np.array(objs, OBJECT_DTYPE), meshes=[sphere, ground],
+                     instance_mesh=[0, 1], instances=inst, texture=tex, config=cfg, seed=seed,
+                     velocities=np.zeros((2, 6), np.float32))
+
+
 def instanced(seed: int = 0x7E55E11A, width=1920, height=1080, spp=8, depth=8, n_instances=1024, tex_size=4096,
               proto_scale=1.0, prototypes=None) -> SceneSpec:
     """Two-level scene: n_instances instances of four prototypes (icosphere
@@ -565,7 +607,8 @@ def mesh_library(meshes: list, builder: Builder):
 def object_bounds(objects: np.ndarray, instances: np.ndarray | None, inst_local: np.ndarray | None,
                   builder: Builder) -> np.ndarray:
     """World AABB per renderable: SphereBounds, QuadBounds, TriangleBounds,
-    MeshBounds (scene/shapes.h:71-145), float32 arithmetic in the same order."""
+    BoxBounds, MeshBounds (scene/shapes.h:71-145), float32 arithmetic in the
+    same order."""
     n = len(objects)
     b = np.zeros(n, AABB_DTYPE)
     f = np.float32
@@ -591,6 +634,18 @@ def object_bounds(objects: np.ndarray, instances: np.ndarray | None, inst_local:
         c0, c1, c2 = P[t], P[t] + U[t], P[t] + V[t]
         lo[t] = smin(smin(c0, c1), c2)
         hi[t] = smax(smax(c0, c1), c2)
+    bx = shape == BOX
+    if bx.any():
+        # BoxBounds (scene/shapes.h:102-112): yaw-rotated half extents
+        yaw = objects["rotation"][bx].astype(f)
+        sn, cs = np.abs(np.sin(yaw)).astype(f), np.abs(np.cos(yaw)).astype(f)
+        hx, hy, hz = (objects["half_x"][bx].astype(f), objects["half_y"][bx].astype(f),
+                      objects["half_z"][bx].astype(f))
+        ex = (cs * hx + sn * hz).astype(f)
+        ez = (sn * hx + cs * hz).astype(f)
+        Pb = P[bx]
+        lo[bx] = np.stack([Pb[:, 0] - ex, Pb[:, 1] - hy, Pb[:, 2] - ez], 1)
+        hi[bx] = np.stack([Pb[:, 0] + ex, Pb[:, 1] + hy, Pb[:, 2] + ez], 1)
     m = np.nonzero(shape == MESH)[0]
     if len(m):
         wb = builder.instance_bounds(instances, inst_local)

@@ -264,13 +264,53 @@ int avrt_upload_textures(avrt_ctx *ctx, const uint8_t *rgba8_srgb,
  * is finished. */
 int avrt_render(avrt_ctx *ctx, const AvrtConfig *config, int mode);
 
+/* Renderer::RenderFrame in hybrid mode (renderer.cpp:1330-1583): the
+ * G-buffer pass, deferred.comp over the trace grid and, below full
+ * resolution, upsample.comp. The G-buffer is ray-cast, not rasterized: for
+ * every pixel the primary ray through the jittered pixel centre is traced
+ * against what the reference rasterizes (meshes and IsRasterProxy boxes,
+ * renderer.h / deferred.comp.hlsl:1011-1017), with gbuffer.frag.hlsl's
+ * outputs and formats (position + slot RGBA32F, normal + coverage RGBA16F,
+ * albedo x texture + view-dependence RGBA8 UNORM, PerspectiveRH_ZO depth).
+ * The config carries the frame's hybrid uniforms as the reference's
+ * RenderFrame fills them (jitter_x/y, traceScale, frameIndex, historyReset,
+ * cameraMoved, historyCap, prevViewProj, displayOnly, debugView, zNear,
+ * zFar): the caller plays the role of Renderer (see avrt_halton). The
+ * context keeps the two history slots (accumulation and per-pixel geometry)
+ * and swaps them after every traced, non-debug frame, as historyIndex does.
+ * Sample semantics: config.samples / config.batches lighting samples per
+ * traced cell per call (deferred.comp.hlsl:1960). Blocks until finished. */
+int avrt_render_hybrid(avrt_ctx *ctx, const AvrtConfig *config);
+
+/* Renderer::Halton (renderer.cpp:1202-1213): radical inverse in float. The
+ * hybrid jitter of frame f is Halton((f % 1024) + 1, 2) - 0.5 and
+ * Halton(.., 3) - 0.5 (renderer.cpp:1389-1391). Host code. */
+float avrt_halton(uint32_t index, uint32_t base);
+
 /* Counters and device times of the last render. */
 int avrt_get_stats(const avrt_ctx *ctx, AvrtStats *out);
 
-/* Read back the accumulation image as RGBA32F (width*height*16 bytes) and the
- * display image as RGBA8 (width*height*4 bytes; hybrid mode only). */
+/* Read back the accumulation image as RGBA32F (width*height*16 bytes; after a
+ * hybrid frame, the history slot it wrote) and the display image as RGBA8
+ * (width*height*4 bytes). */
 int avrt_read_image(avrt_ctx *ctx, float *dst, size_t bytes);
 int avrt_read_display(avrt_ctx *ctx, uint8_t *dst, size_t bytes);
+
+/* The G-buffer of the last hybrid frame, `pixels` = width*height entries per
+ * target: position xyz + object slot (4 floats), normal xyz + coverage (4
+ * IEEE halves), albedo rgb + (0 if METAL/DIELECTRIC else 255) (4 bytes),
+ * depth (float, 1 where nothing was drawn). Any pointer may be null. */
+int avrt_read_gbuffer(avrt_ctx *ctx, float *position, uint16_t *normal_half,
+                      uint8_t *albedo, float *depth, size_t pixels);
+
+/* The history slot written by the last traced hybrid frame: accumulation
+ * (RGBA) and geometry (position xyz, +-history length: negative where the
+ * pixel saw sky), `pixels` entries of 4 floats each. */
+int avrt_read_history(avrt_ctx *ctx, float *image, float *geometry, size_t pixels);
+
+/* The reduced-grid lighting of the last hybrid frame (RGBA32F per cell:
+ * colour, divided by the albedo where alpha is 1), `cells` entries. */
+int avrt_read_lighting_reduced(avrt_ctx *ctx, float *dst, size_t cells);
 
 /* Primary visibility only: for sample `config.batch` of every pixel, the TLAS
  * slot of the closest object (0xFFFFFFFF on a miss), the mesh triangle slot

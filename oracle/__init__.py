@@ -32,6 +32,11 @@ _SIG = {
                                      _U32, _U32, _U32, ctypes.c_int, ctypes.c_int, _P, _P, _P, _U32]),
     "orc_closest_fp64": (None, [_P, _U32, _P, _U32, _P, _P, _P, _P, _P, _P]),
     "orc_sizeof_config": (ctypes.c_int, []),
+    "orc_halton": (ctypes.c_float, [_U32, _U32]),
+    "orc_exp": (ctypes.c_float, [ctypes.c_float]),
+    "orc_gbuffer": (None, [_P, _P, _U32, _P, _U32, _P, _P, _P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P]),
+    "orc_hybrid_frame": (None, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _P, _P, _P, _P, _U32, _U32, _U32, _P, _P, _P,
+                                _P, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int]),
 }
 
 _lib = None
@@ -121,3 +126,54 @@ def closest_fp64(packed, rays):
     lib.orc_closest_fp64(_p(rays), n, _p(packed.objects), len(packed.objects), _p(packed.vertices),
                          _p(packed.indices), _p(packed.instances), _p(obj), _p(prim), _p(t))
     return obj, prim, t
+
+
+def halton(index, base):
+    return float(load().orc_halton(int(index), int(base)))
+
+
+def gbuffer(packed, config):
+    """Ray-cast G-buffer: (position+slot [H,W,4] f32, normal+coverage [H,W,4]
+    f32 holding FP16-rounded values, albedo+flag [H,W,4] u8, depth [H,W] f32)."""
+    lib = load()
+    cfg = np.array(config, copy=True).reshape(1)
+    W, H = int(cfg["width"][0]), int(cfg["height"][0])
+    pos = np.zeros((H, W, 4), np.float32)
+    nrm = np.zeros((H, W, 4), np.float32)
+    alb = np.zeros((H, W, 4), np.uint8)
+    dep = np.zeros((H, W), np.float32)
+    t, tw, th, tl = _geometry(packed)
+    lib.orc_gbuffer(_p(cfg), _p(packed.objects), len(packed.objects), _p(packed.tlas), len(packed.tlas),
+                    _p(packed.vertices), _p(packed.indices), _p(packed.blas), _p(packed.instances), _p(t), tw, th, tl,
+                    _p(pos), _p(nrm), _p(alb), _p(dep))
+    return pos, nrm, alb, dep
+
+
+def hybrid_frame(packed, config, gbuf, prev_img, prev_geom, fp16=False, threads=None):
+    """deferred.comp (+ upsample.comp below full resolution) for one frame.
+    Returns (img, geom, display, lighting_reduced)."""
+    lib = load()
+    cfg = np.array(config, copy=True).reshape(1)
+    W, H = int(cfg["width"][0]), int(cfg["height"][0])
+    scale = float(cfg["traceScale"][0])
+    reduced = 0.0 < scale < 1.0
+    tw = int(np.ceil(np.float32(W) * np.float32(scale))) if reduced else W
+    th = int(np.ceil(np.float32(H) * np.float32(scale))) if reduced else H
+    pos, nrm, alb, _ = gbuf
+    pos = np.ascontiguousarray(pos, np.float32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    alb = np.ascontiguousarray(alb, np.uint8)
+    prev_img = np.ascontiguousarray(prev_img, np.float32)
+    prev_geom = np.ascontiguousarray(prev_geom, np.float32)
+    img = np.zeros((H, W, 4), np.float32)
+    geom = np.zeros((H, W, 4), np.float32)
+    disp = np.zeros((H, W, 4), np.uint8)
+    lred = np.zeros((th, tw, 4), np.float32)
+    t, tw_, th_, tl = _geometry(packed)
+    threads = threads or os.cpu_count() or 1
+    lib.orc_hybrid_frame(_p(cfg), _p(packed.objects), len(packed.objects), _p(packed.tlas), len(packed.tlas),
+                         _p(packed.lights), len(packed.lights), _p(packed.vertices), _p(packed.indices),
+                         _p(packed.blas), _p(packed.instances), _p(t), tw_, th_, tl, _p(pos), _p(nrm), _p(alb),
+                         _p(prev_img), _p(prev_geom), _p(img), _p(geom), _p(disp), _p(lred), 1 if fp16 else 0,
+                         int(threads))
+    return img, geom, disp, lred

@@ -20,6 +20,12 @@
 //                                                             ray_trace.comp.hybrid.hlsl:940-1325
 //   main (camera, accumulation, ACES, sRGB)                   ray_trace.comp.hybrid.hlsl:1328-1404
 //   path_trace main / GetBounce / SurfaceEmission             shaders/path_trace.comp.hlsl:675-860
+//   G-buffer outputs (ray-cast; see orc_gbuffer)              shaders/gbuffer.frag.hlsl:96-122
+//   TraverseBVHBounded / Occluded / OccludedMesh / HitFromGBuffer / ResolvePrimary /
+//   ClampedDirectLight / TraceIndirect / TracedTexel / PreviousPosition /
+//   ReprojectHistory / main                                   shaders/deferred.comp.hlsl:684-2010
+//   upsample main                                             shaders/upsample.comp.hlsl:280-390
+//   Halton                                                    renderer/renderer.cpp:1202-1213
 //
 // Choices where HLSL leaves the result open (each is shared with the HIP
 // path and stated here so the oracle stands alone):
@@ -37,6 +43,14 @@
 //     definition (the reference's Vulkan sampler cannot be run here).
 //   * Volumes (ISOTROPIC) are not restated: they draw random numbers inside
 //     traversal, so their streams depend on traversal order.
+//   * exp in upsample.comp is rt_exp below (Cody-Waite reduction, Taylor
+//     degree 7); pow(x, 32) is five squarings; sin/cos of a yaw in
+//     PreviousPosition use rt_sincos.
+//   * The G-buffer is the ray-cast equivalent of the raster pass: per pixel
+//     the closest hit of the jittered primary ray among meshes and raster-proxy
+//     boxes (no clip planes); normal = the surface's own unit normal (mesh:
+//     interpolated vertex normal, rotated), stored through FP16; albedo =
+//     SurfaceAlbedo through 8-bit UNORM, read back as byte / 255.
 //
 // All entry points are extern "C" with plain pointers, for ctypes.
 
@@ -1191,13 +1205,232 @@ static V3 previous_position(const Scene &s, const Object &o, V3 p)
         rx = in.qx; ry = in.qy; rz = in.qz; rw = in.qw;
     } else if (o.shapeType == SPHERE || o.shapeType == BOX) {
         const float h = 0.5f * o.rotation;
-        ry = std::sin(h);
-        rw = std::cos(h);
+        rt_sincos(h, &ry, &rw);
     }
     const V3 local = quat_rotate(-rx, -ry, -rz, rw, p - pos_of(o));
     const V3 pv = v3(o.prevRotX, o.prevRotY, o.prevRotZ);
     const float pw = std::sqrt(fmaxn(0.0f, 1.0f - dot(pv, pv)));
     return quat_rotate(pv.x, pv.y, pv.z, pw, local) + pos2_of(o);
+}
+
+static void hybrid_params_of(const Config &cfg, HybridParams &hp, uint32_t *tw, uint32_t *th, bool *reduced,
+                             uint32_t *spf)
+{
+    hp.jitter_x = cfg.jitter_x;
+    hp.jitter_y = cfg.jitter_y;
+    hp.trace_scale = (cfg.traceScale > 0.0f && cfg.traceScale < 1.0f) ? cfg.traceScale : 1.0f;
+    hp.frame_index = cfg.frameIndex;
+    hp.history_reset = cfg.historyReset;
+    hp.camera_moved = cfg.cameraMoved;
+    hp.history_cap = cfg.historyCap;
+    std::memcpy(hp.prev_view_proj, cfg.prevViewProj, sizeof(hp.prev_view_proj));
+    hp.tan_half_fov = std::tan((cfg.fov / 2.0f) * 0.0174532925f);
+    *reduced = hp.trace_scale < 1.0f;
+    *tw = *reduced ? static_cast<uint32_t>(std::ceil(static_cast<float>(cfg.width) * hp.trace_scale)) : cfg.width;
+    *th = *reduced ? static_cast<uint32_t>(std::ceil(static_cast<float>(cfg.height) * hp.trace_scale)) : cfg.height;
+    const uint32_t b = cfg.batches ? cfg.batches : 1u;
+    *spf = std::max(cfg.samples / b, 1u);
+}
+
+static inline Ray jittered_ray(const Camera &cam, const HybridParams &hp, uint32_t x, uint32_t y)
+{
+    const V3 ps = cam.pixel0 + (static_cast<float>(x) + hp.jitter_x) * cam.pixelU +
+                  (static_cast<float>(y) + hp.jitter_y) * cam.pixelV;
+    Ray r;
+    r.o = cam.source;
+    r.d = normalize(ps - cam.source);
+    r.time = 0.0f;
+    return r;
+}
+
+static inline float unorm_read(uint8_t b) { return static_cast<float>(b) / 255.0f; }
+
+// One traced cell of deferred.comp main (:1880-2009). Writes the reduced
+// lighting (reduced grid) or the pixel's accumulation, geometry and display.
+struct HybridIO {
+    const float *gpos, *gnrm;
+    const uint8_t *galb;
+    const float *prev_img, *prev_geom;
+    float *img, *geom, *lred;
+    uint8_t *display;
+    int fp16;
+};
+
+static void deferred_cell(const Scene &s, const Config &cfg, const HybridParams &hp, const Camera &cam, bool reduced,
+                          uint32_t spf, uint32_t tw, uint32_t cx, uint32_t cy, const HybridIO &io)
+{
+    const uint32_t W = cfg.width, H = cfg.height;
+    uint32_t ix = cx, iy = cy;
+    if (reduced) traced_texel(cx, cy, hp.frame_index, hp.trace_scale, W, H, &ix, &iy);
+    Rng rng{ix + iy * W + hp.frame_index * W * H + 1u};
+    Ray primary = jittered_ray(cam, hp, ix, iy);
+    primary.time = rng.next();
+    const V3 src = cam.source;
+    const V3 sky = v3(cfg.sky[0], cfg.sky[1], cfg.sky[2]);
+    const size_t pix = static_cast<size_t>(iy) * W + ix;
+    // HitFromGBuffer
+    Hit raster{};
+    raster.pos = v3(0.0f, 0.0f, 0.0f);
+    raster.normal = v3(0.0f, 0.0f, 0.0f);
+    raster.u = raster.v = 0.0f;
+    raster.face = false;
+    raster.obj = 0;
+    raster.prim = ~0u;
+    V3 raster_albedo = v3(0.0f, 0.0f, 0.0f);
+    const bool has_raster = !(io.gnrm[4 * pix + 3] < 0.5f);
+    if (has_raster) {
+        raster.pos = v3(io.gpos[4 * pix], io.gpos[4 * pix + 1], io.gpos[4 * pix + 2]);
+        raster.obj = static_cast<uint32_t>(io.gpos[4 * pix + 3] + 0.5f);
+        raster.normal = normalize(v3(io.gnrm[4 * pix], io.gnrm[4 * pix + 1], io.gnrm[4 * pix + 2]));
+        raster.t = length(raster.pos - src);
+        raster_albedo = v3(unorm_read(io.galb[4 * pix]), unorm_read(io.galb[4 * pix + 1]), unorm_read(io.galb[4 * pix + 2]));
+        raster.face = dot(primary.d, raster.normal) < 0.0f;
+        if (!raster.face) raster.normal = -raster.normal;
+    }
+    // ResolvePrimary
+    const float far_limit = has_raster ? raster.t : 1000000.0f;
+    Hit ah;
+    const bool analytic = traverse_bounded(s, primary, 0.001f, far_limit, 2, ah);
+    Surface sf;
+    uint32_t source;
+    V3 alb_flat;
+    if (analytic) {
+        sf.hit = ah;
+        sf.color_type = s.objects[ah.obj].colorType;
+        sf.albedo = surface_albedo_hybrid(s, ah);
+        sf.from_gbuffer = false;
+        alb_flat = albedo_of(s.objects[ah.obj]);
+        source = 2;
+    } else {
+        sf.hit = raster;
+        sf.color_type = has_raster ? s.objects[raster.obj].colorType : LAMBERTIAN;
+        sf.albedo = raster_albedo;
+        sf.from_gbuffer = true;
+        alb_flat = raster_albedo;
+        source = has_raster ? 1 : 0;
+    }
+    V3 color = v3(0.0f, 0.0f, 0.0f);
+    if (source == 0) {
+        color = sky;
+    } else if (sf.color_type == DIFFUSE_LIGHT) {
+        color = s.objects[sf.hit.obj].emission * alb_flat;
+    } else {
+        for (uint32_t k = 0; k < spf; k++) {
+            if (sf.color_type == LAMBERTIAN) color = color + clamped_direct_deferred(s, sf.hit, sf.albedo, primary, rng, nullptr);
+            color = color + trace_indirect(s, primary, sf, cfg.depth, sky, rng, nullptr);
+        }
+        color = color / static_cast<float>(spf);
+    }
+    auto store = [&](float x) { return io.fp16 ? round_half(x) : x; };
+    if (reduced) {
+        const bool demod = source == 1 && sf.color_type != DIELECTRIC;
+        V3 a = v3(1.0f, 1.0f, 1.0f);
+        if (demod) a = v3(fmaxn(sf.albedo.x, 0.01f), fmaxn(sf.albedo.y, 0.01f), fmaxn(sf.albedo.z, 0.01f));
+        float *o = io.lred + 4 * (static_cast<size_t>(cy) * tw + cx);
+        o[0] = store(color.x / a.x);
+        o[1] = store(color.y / a.y);
+        o[2] = store(color.z / a.z);
+        o[3] = demod ? 1.0f : 0.0f;
+        return;
+    }
+    const bool surface = source != 0;
+    const bool view_dep = surface && (sf.color_type == METAL || sf.color_type == DIELECTRIC);
+    const V3 pos = sf.hit.pos;
+    const V3 prev_pos = surface ? previous_position(s, s.objects[sf.hit.obj], pos) : pos;
+    V3 acc;
+    float count;
+    reproject(cfg, hp, ix, iy, color, surface, pos, prev_pos, sf.hit.normal, view_dep, io.prev_img, io.prev_geom, &acc,
+              &count);
+    io.img[4 * pix] = store(acc.x);
+    io.img[4 * pix + 1] = store(acc.y);
+    io.img[4 * pix + 2] = store(acc.z);
+    io.img[4 * pix + 3] = 1.0f;
+    io.geom[4 * pix] = pos.x;
+    io.geom[4 * pix + 1] = pos.y;
+    io.geom[4 * pix + 2] = pos.z;
+    io.geom[4 * pix + 3] = surface ? count : -count;
+    const float e = cfg.exposure;
+    io.display[4 * pix] = unorm8(to_srgb(aces(acc.x * e)));
+    io.display[4 * pix + 1] = unorm8(to_srgb(aces(acc.y * e)));
+    io.display[4 * pix + 2] = unorm8(to_srgb(aces(acc.z * e)));
+    io.display[4 * pix + 3] = 255;
+}
+
+// upsample.comp main (:280-390) for pixel (px, py).
+static void upsample_pixel(const Scene &s, const Config &cfg, const HybridParams &hp, uint32_t tw, uint32_t th,
+                           uint32_t px, uint32_t py, const HybridIO &io)
+{
+    const uint32_t W = cfg.width, H = cfg.height;
+    const V3 src = v3(cfg.source[0], cfg.source[1], cfg.source[2]);
+    const size_t p = static_cast<size_t>(py) * W + px;
+    const int cmx = static_cast<int>(tw) - 1, cmy = static_cast<int>(th) - 1;
+    const V3 normal_p = v3(io.gnrm[4 * p], io.gnrm[4 * p + 1], io.gnrm[4 * p + 2]);
+    const bool covered_p = io.gnrm[4 * p + 3] > 0.5f;
+    const V3 position_p = v3(io.gpos[4 * p], io.gpos[4 * p + 1], io.gpos[4 * p + 2]);
+    const float depth_p = length(position_p - src);
+    const V3 albedo_p = v3(unorm_read(io.galb[4 * p]), unorm_read(io.galb[4 * p + 1]), unorm_read(io.galb[4 * p + 2]));
+    const float cpx = (static_cast<float>(px) + 0.5f) * hp.trace_scale - 0.5f;
+    const float cpy = (static_cast<float>(py) + 0.5f) * hp.trace_scale - 0.5f;
+    const int bx = static_cast<int>(std::floor(cpx)), by = static_cast<int>(std::floor(cpy));
+    V3 sum = v3(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    V3 nearest = v3(0.0f, 0.0f, 0.0f);
+    float nearest_d = 1e30f;
+    for (int j = 0; j < 2; j++) {
+        for (int i = 0; i < 2; i++) {
+            const int cx = std::min(std::max(bx + i, 0), cmx), cy = std::min(std::max(by + j, 0), cmy);
+            uint32_t tx, ty;
+            traced_texel(static_cast<uint32_t>(cx), static_cast<uint32_t>(cy), hp.frame_index, hp.trace_scale, W, H, &tx, &ty);
+            const float *lt = io.lred + 4 * (static_cast<size_t>(cy) * tw + cx);
+            const size_t t = static_cast<size_t>(ty) * W + tx;
+            const bool covered_t = io.gnrm[4 * t + 3] > 0.5f;
+            V3 alb = albedo_p;
+            if (!covered_p) alb = v3(unorm_read(io.galb[4 * t]), unorm_read(io.galb[4 * t + 1]), unorm_read(io.galb[4 * t + 2]));
+            const V3 lrgb = v3(lt[0], lt[1], lt[2]);
+            const V3 radiance = (lt[3] > 0.5f) ? lrgb * alb : lrgb;
+            const float ox = static_cast<float>(tx) - static_cast<float>(px);
+            const float oy = static_cast<float>(ty) - static_cast<float>(py);
+            const float d2 = ox * ox + oy * oy;
+            if (d2 < nearest_d) {
+                nearest_d = d2;
+                nearest = radiance;
+            }
+            float w = rt_exp((-d2 * hp.trace_scale) * hp.trace_scale);
+            if (covered_t != covered_p) {
+                w = 0.0f;
+            } else if (covered_p) {
+                const V3 pt = v3(io.gpos[4 * t], io.gpos[4 * t + 1], io.gpos[4 * t + 2]);
+                const float depth_t = length(pt - src);
+                w = w * rt_exp(-std::fabs(depth_p - depth_t) / (0.01f * depth_p + 0.001f));
+                const V3 normal_t = v3(io.gnrm[4 * t], io.gnrm[4 * t + 1], io.gnrm[4 * t + 2]);
+                w = w * pow32(saturate(dot(normal_p, normal_t)));
+            }
+            sum = sum + w * radiance;
+            wsum = wsum + w;
+        }
+    }
+    const V3 color = (wsum > 1e-4f) ? sum / wsum : nearest;
+    const bool view_dep = covered_p && unorm_read(io.galb[4 * p + 3]) < 0.5f;
+    const V3 prev_pos = covered_p ? previous_position(s, s.objects[static_cast<uint32_t>(io.gpos[4 * p + 3])], position_p)
+                                  : position_p;
+    V3 acc;
+    float count;
+    reproject(cfg, hp, px, py, color, covered_p, position_p, prev_pos, normal_p, view_dep, io.prev_img, io.prev_geom, &acc,
+              &count);
+    auto store = [&](float x) { return io.fp16 ? round_half(x) : x; };
+    io.img[4 * p] = store(acc.x);
+    io.img[4 * p + 1] = store(acc.y);
+    io.img[4 * p + 2] = store(acc.z);
+    io.img[4 * p + 3] = 1.0f;
+    io.geom[4 * p] = position_p.x;
+    io.geom[4 * p + 1] = position_p.y;
+    io.geom[4 * p + 2] = position_p.z;
+    io.geom[4 * p + 3] = covered_p ? count : -count;
+    const float e = cfg.exposure;
+    io.display[4 * p] = unorm8(to_srgb(aces(acc.x * e)));
+    io.display[4 * p + 1] = unorm8(to_srgb(aces(acc.y * e)));
+    io.display[4 * p + 2] = unorm8(to_srgb(aces(acc.z * e)));
+    io.display[4 * p + 3] = 255;
 }
 } // namespace orc
 
@@ -1669,6 +1902,118 @@ void orc_closest_fp64(const float *rays, uint32_t nrays, const Object *objects, 
         hit_obj[k] = bo;
         hit_prim[k] = bp;
         hit_t[k] = bo == ~0u ? 0.0 : best;
+    }
+}
+
+float orc_halton(uint32_t index, uint32_t base)
+{
+    float result = 0.0f;
+    float fraction = 1.0f;
+    while (index > 0) {
+        fraction /= static_cast<float>(base);
+        result += fraction * static_cast<float>(index % base);
+        index /= base;
+    }
+    return result;
+}
+
+float orc_exp(float x) { return rt_exp(x); }
+
+// Ray-cast G-buffer of a hybrid frame: position + object slot (4 floats),
+// normal + coverage (4 floats, the normal rounded through FP16), albedo +
+// view-dependence (4 bytes), depth (1 float; 1 where nothing is drawn).
+void orc_gbuffer(const Config *cfg, const Object *objects, uint32_t nobj, const Node *bvh, uint32_t nnode,
+                 const Vertex *verts, const uint32_t *idx, const Node *blas, const Inst *inst, const uint8_t *tex,
+                 uint32_t tw, uint32_t th, uint32_t tl, float *gpos, float *gnrm, uint8_t *galb, float *gdep)
+{
+    Scene s = make_scene(objects, nobj, bvh, nnode, nullptr, 0, verts, idx, blas, inst, tex, tw, th, tl);
+    const Camera cam = make_camera(*cfg);
+    HybridParams hp;
+    uint32_t gw, gh, spf;
+    bool reduced;
+    hybrid_params_of(*cfg, hp, &gw, &gh, &reduced, &spf);
+    const uint32_t W = cfg->width, H = cfg->height;
+    V3 w = normalize(cam.source - v3(cfg->target[0], cfg->target[1], cfg->target[2]));
+    const float wd = w.x * cfg->source[0] + w.y * cfg->source[1] + w.z * cfg->source[2];
+    const float m22 = cfg->zFar / (cfg->zNear - cfg->zFar);
+    const float m23 = (cfg->zNear * cfg->zFar) / (cfg->zNear - cfg->zFar);
+    for (uint32_t y = 0; y < H; y++) {
+        for (uint32_t x = 0; x < W; x++) {
+            const size_t p = static_cast<size_t>(y) * W + x;
+            const Ray r = jittered_ray(cam, hp, x, y);
+            Hit h;
+            if (!traverse_bounded(s, r, 0.001f, 1000000.0f, 1, h)) {
+                for (int k = 0; k < 4; k++) {
+                    gpos[4 * p + k] = 0.0f;
+                    gnrm[4 * p + k] = 0.0f;
+                    galb[4 * p + k] = 0;
+                }
+                gdep[p] = 1.0f;
+                continue;
+            }
+            const Object &o = s.objects[h.obj];
+            const V3 n = h.face ? h.normal : -h.normal;
+            const V3 a = surface_albedo_hybrid(s, h);
+            gpos[4 * p] = h.pos.x;
+            gpos[4 * p + 1] = h.pos.y;
+            gpos[4 * p + 2] = h.pos.z;
+            gpos[4 * p + 3] = static_cast<float>(h.obj);
+            gnrm[4 * p] = round_half(n.x);
+            gnrm[4 * p + 1] = round_half(n.y);
+            gnrm[4 * p + 2] = round_half(n.z);
+            gnrm[4 * p + 3] = 1.0f;
+            galb[4 * p] = unorm8(a.x);
+            galb[4 * p + 1] = unorm8(a.y);
+            galb[4 * p + 2] = unorm8(a.z);
+            galb[4 * p + 3] = (o.colorType == METAL || o.colorType == DIELECTRIC) ? 0 : 255;
+            const float zv = dot(w, h.pos) - wd;
+            gdep[p] = (m22 * zv + m23) / (-zv);
+        }
+    }
+}
+
+// deferred.comp over the trace grid, then (reduced grid) upsample.comp, for
+// one hybrid frame given its G-buffer (as orc_gbuffer writes it) and the
+// history the frame reads. Writes the history it produces (img, geom: 4
+// floats per pixel), the display (4 bytes per pixel) and the reduced
+// lighting (4 floats per cell, reduced grid only). Rows of cells are spread
+// over `threads` workers.
+void orc_hybrid_frame(const Config *cfg, const Object *objects, uint32_t nobj, const Node *bvh, uint32_t nnode,
+                      const uint32_t *lights, uint32_t nlights, const Vertex *verts, const uint32_t *idx,
+                      const Node *blas, const Inst *inst, const uint8_t *tex, uint32_t tw, uint32_t th, uint32_t tl,
+                      const float *gpos, const float *gnrm, const uint8_t *galb, const float *prev_img,
+                      const float *prev_geom, float *img, float *geom, uint8_t *display, float *lred, int fp16,
+                      int threads)
+{
+    Scene s = make_scene(objects, nobj, bvh, nnode, lights, nlights, verts, idx, blas, inst, tex, tw, th, tl);
+    const Camera cam = make_camera(*cfg);
+    HybridParams hp;
+    uint32_t gw, gh, spf;
+    bool reduced;
+    hybrid_params_of(*cfg, hp, &gw, &gh, &reduced, &spf);
+    HybridIO io{gpos, gnrm, galb, prev_img, prev_geom, img, geom, lred, display, fp16};
+    const int nt = threads > 0 ? threads : 1;
+    auto run_rows = [&](uint32_t rows, auto &&fn) {
+        std::atomic<uint32_t> next{0};
+        auto worker = [&]() {
+            while (true) {
+                const uint32_t y = next.fetch_add(1);
+                if (y >= rows) break;
+                fn(y);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(worker);
+        worker();
+        for (auto &t : pool) t.join();
+    };
+    run_rows(gh, [&](uint32_t cy) {
+        for (uint32_t cx = 0; cx < gw; cx++) deferred_cell(s, *cfg, hp, cam, reduced, spf, gw, cx, cy, io);
+    });
+    if (reduced) {
+        run_rows(cfg->height, [&](uint32_t py) {
+            for (uint32_t px = 0; px < cfg->width; px++) upsample_pixel(s, *cfg, hp, gw, gh, px, py, io);
+        });
     }
 }
 
