@@ -105,6 +105,28 @@ class AvrtStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class AvrtServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("max_jobs", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("round_paths", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class AvrtJobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("samples_done", ctypes.c_uint32), ("samples", ctypes.c_uint32),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class AvrtServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("rays_primary", ctypes.c_uint64), ("rays_bounce", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64),
+                ("paths", ctypes.c_uint64), ("hbm_used_bytes", ctypes.c_uint64), ("queued", ctypes.c_uint32),
+                ("active", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+JOB_QUEUED, JOB_ACTIVE, JOB_DONE = 0, 1, 2
+
 # Every entry point include/avrt.h declares: name -> (restype, argtypes).
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -135,6 +157,16 @@ _SIG = {
     "avrt_update_vertices": (ctypes.c_int, [_P, _P, _U32, _U32]),
     "avrt_read_blas": (ctypes.c_int, [_P, _P, _U32]),
     "avrt_version": (ctypes.c_char_p, []),
+    "avrt_server_create": (ctypes.c_int, [ctypes.POINTER(AvrtServerOptions), ctypes.POINTER(_P)]),
+    "avrt_server_destroy": (ctypes.c_int, [_P]),
+    "avrt_server_last_error": (ctypes.c_char_p, [_P]),
+    "avrt_server_add_scene": (ctypes.c_int, [_P, _P, ctypes.POINTER(_U32)]),
+    "avrt_server_submit": (ctypes.c_int, [_P, _U32, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "avrt_server_step": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
+    "avrt_server_job_status": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(AvrtJobStatus)]),
+    "avrt_server_read_job": (ctypes.c_int, [_P, ctypes.c_uint64, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "avrt_server_release_job": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "avrt_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(AvrtServerStats)]),
 }
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))

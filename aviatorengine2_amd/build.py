@@ -17,7 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libavrt.so")
 
-SOURCES = ["avrt_render.hip", "avrt_hybrid.hip", "avrt_scene.hip", "scene_convert.cpp", "bvh_host.cpp"]
+SOURCES = ["avrt_render.hip", "avrt_hybrid.hip", "avrt_serve.hip", "avrt_scene.hip", "scene_convert.cpp", "bvh_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",

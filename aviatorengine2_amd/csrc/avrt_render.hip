@@ -46,231 +46,33 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t firs
 {
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= n) return;
-    const uint32_t P = fp.width * fp.height;
-    const uint32_t sidx = pid / P;
-    const uint32_t p = pid - sidx * P;
-    const uint32_t x = p % fp.width, y = p / fp.width;
-    const uint32_t batch = fp.batch + first_sample + sidx;
-    Rng rng{x + y * fp.width + batch * fp.width * fp.height + 1u};
-    const float jx = rng.next() - 0.5f;
-    const float jy = rng.next() - 0.5f;
-    const F3 p0 = ld3(fp.pixel0), pu = ld3(fp.pixelU), pv = ld3(fp.pixelV);
-    const F3 target = p0 + (static_cast<float>(x) + jx) * pu + (static_cast<float>(y) + jy) * pv;
-    F3 o = ld3(fp.source);
-    if (!(fp.defocus <= 0.0f)) {
-        float px, py;
-        while (true) {
-            px = rng.next() * 2.0f - 1.0f;
-            py = rng.next() * 2.0f - 1.0f;
-            if (sqrtf(px * px + py * py + 0.0f * 0.0f) < 1.0f) break;
-        }
-        o = ld3(fp.source) + (px * ld3(fp.du)) + (py * ld3(fp.dv));
-    }
-    const F3 d = normalize(target - o);
-    const float time = rng.next();
-    out.o[pid] = make_float4(o.x, o.y, o.z, time);
-    out.d[pid] = make_float4(d.x, d.y, d.z, __uint_as_float(pid));
-    out.t[pid] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(rng.s));
-    Lpath[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    raygen_path(fp, first_sample, pid, pid, out, Lpath);
 }
 
-// MODE 0 = path_trace.comp, MODE 1 = ray_trace.comp.hybrid. WAVE_NEE: sample
-// the lights here and emit shadow rays for k_trace (up to kWaveLights lights);
-// otherwise store a record that k_shadow resolves in one kernel.
 template <int MODE, bool WAVE_NEE>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, uint32_t n, uint32_t bounce,
                                                   PathBufs in, const uint4 *__restrict__ hitA,
                                                   const float *__restrict__ hitB, PathBufs out, ShadowBufs sh,
                                                   ShadowRays sr, NeeBufs nee, float4 *Lpath, uint32_t *counters)
 {
-    F3 lpt[kWaveLights], ldir[kWaveLights], lc[kWaveLights];
-    bool lray[kWaveLights];
-    for (uint32_t i = 0; i < kWaveLights; i++) {
-        lray[i] = false;
-        lpt[i] = f3(0, 0, 0);
-        ldir[i] = f3(0, 0, 0);
-        lc[i] = f3(0, 0, 0);
-    }
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    bool cont = false;
-    bool want_shadow = false;
-    F3 no = f3(0, 0, 0), nd = f3(0, 0, 0), ntp = f3(0, 0, 0);
-    float time = 0.0f;
-    uint32_t pid = 0, nrng = 0;
-    Surf s;
-    s.pos = f3(0, 0, 0);
-    s.normal = f3(0, 0, 0);
-    F3 albedo = f3(0, 0, 0), tp_at_hit = f3(0, 0, 0);
-    uint32_t rng_at_nee = 0;
-    if (k < n) {
-        const float4 O = in.o[k];
-        const float4 D = in.d[k];
-        const float4 T = in.t[k];
-        const uint4 HA = hitA[k];
-        const F3 o = f3(O.x, O.y, O.z), d = f3(D.x, D.y, D.z);
-        F3 tp = f3(T.x, T.y, T.z);
-        time = O.w;
-        pid = __float_as_uint(D.w);
-        Rng rng{__float_as_uint(T.w)};
-        HitRec h;
-        h.t = __uint_as_float(HA.x);
-        h.obj = HA.y;
-        h.prim = HA.z;
-        h.alpha = __uint_as_float(HA.w);
-        h.beta = hitB[k];
-        if (h.obj == ~0u) {
-            if (MODE == 0) {
-                const float t = 0.5f * (normalize(d).y + 1.0f);
-                add_radiance(Lpath, pid, tp * lerp(ld3(fp.horizon), ld3(fp.sky), t));
-            } else {
-                add_radiance(Lpath, pid, tp * ld3(fp.sky));
-            }
-        } else {
-            const AvrtObject ob = sc.objects[h.obj];
-            hit_attributes(sc, ob, h, o, d, time, s);
-            const F3 alb = f3(ob.r, ob.g, ob.b);
-            if (MODE == 0) {
-                F3 att = alb;
-                if (ob.textureID != AVRT_NO_TEXTURE) att = att * texture_color(sc, fp, ob.textureID, s.u, s.v);
-                if (ob.colorType == AVRT_DIFFUSE_LIGHT) add_radiance(Lpath, pid, tp * (att * ob.emission));
-                F3 b;
-                if (get_bounce(ob, d, s, rng, &b)) {
-                    no = s.pos;
-                    nd = b;
-                    ntp = tp * att;
-                    cont = (bounce + 1 < fp.depth);
-                }
-            } else {
-                if (ob.colorType == AVRT_DIFFUSE_LIGHT) {
-                    add_radiance(Lpath, pid, tp * (ob.emission * alb));
-                } else {
-                    F3 salb = alb;
-                    if (ob.textureID != AVRT_NO_TEXTURE) {
-                        const F3 texel = texture_color(sc, fp, ob.textureID, s.u, s.v);
-                        salb = texel * lerp(f3(1.0f, 1.0f, 1.0f), alb, ob.textureTint);
-                    }
-                    if (ob.colorType == AVRT_LAMBERTIAN && fp.num_lights > 0) {
-                        want_shadow = true;
-                        albedo = salb;
-                        tp_at_hit = tp;
-                        rng_at_nee = rng.s;
-                        if (WAVE_NEE) {
-                            // SampleLight and the early-outs of CalculateDirectLight
-                            // (:1130-1149); the occlusion test is deferred to k_trace.
-                            for (uint32_t i = 0; i < kWaveLights; i++) {
-                                if (i >= fp.num_lights) break;
-                                float dist;
-                                lray[i] = nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lpt[i], &ldir[i],
-                                                    &dist, &lc[i]);
-                            }
-                        } else {
-                            rng.s = rng.s * fp.jumpA + fp.jumpC;
-                        }
-                    }
-                    F3 sc_dir;
-                    if (get_bounce(ob, d, s, rng, &sc_dir)) {
-                        const F3 nd2 = normalize(sc_dir);
-                        const float sign = (dot(nd2, s.normal) < 0.0f) ? -1.0f : 1.0f;
-                        const F3 no2 = s.pos + s.normal * (0.001f * sign);
-                        if (ob.colorType != AVRT_DIELECTRIC) tp = tp * salb;
-                        bool alive = true;
-                        if (bounce > 0) {
-                            float p = fmaxf(tp.x, fmaxf(tp.y, tp.z));
-                            p = fminf(fmaxf(p, 0.1f), 1.0f);
-                            if (rng.next() > p) alive = false;
-                            else tp = f3(tp.x / p, tp.y / p, tp.z / p);
-                        }
-                        if (alive) {
-                            no = no2;
-                            nd = nd2;
-                            ntp = tp;
-                            cont = (bounce + 1 < fp.depth);
-                        }
-                    }
-                }
-            }
-        }
-        nrng = rng.s;
-    }
-    const uint32_t slot = wave_append(cont, &counters[0]);
-    if (cont) {
-        out.o[slot] = make_float4(no.x, no.y, no.z, time);
-        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(pid));
-        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(nrng));
-    }
-    if (MODE == 1 && !WAVE_NEE) {
-        const uint32_t ss = wave_append(want_shadow, &counters[1]);
-        if (want_shadow) {
-            sh.p[ss] = make_float4(s.pos.x, s.pos.y, s.pos.z, time);
-            sh.n[ss] = make_float4(s.normal.x, s.normal.y, s.normal.z, __uint_as_float(pid));
-            sh.a[ss] = make_float4(albedo.x, albedo.y, albedo.z, __uint_as_float(rng_at_nee));
-            sh.w[ss] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, 0.0f);
-        }
-    }
-    if (MODE == 1 && WAVE_NEE) {
-        const uint32_t rec = wave_append(want_shadow, &counters[1]);
-        const uint32_t nl = fp.num_lights;
-        if (want_shadow) {
-            nee.tp[rec] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, __uint_as_float(pid));
-            for (uint32_t i = 0; i < kWaveLights; i++)
-                if (i < nl) nee.contrib[rec * nl + i] = make_float4(lc[i].x, lc[i].y, lc[i].z, 0.0f);
-        }
-        const F3 so = s.pos + (s.normal * 0.001f);
-        for (uint32_t i = 0; i < kWaveLights; i++) {
-            const bool emit = want_shadow && lray[i];
-            const uint32_t r = wave_append(emit, &counters[2]);
-            if (emit) {
-                sr.o[r] = make_float4(so.x, so.y, so.z, time);
-                sr.d[r] = make_float4(ldir[i].x, ldir[i].y, ldir[i].z, __uint_as_float(rec * nl + i));
-                sr.lp[r] = make_float4(lpt[i].x, lpt[i].y, lpt[i].z, __uint_as_float(0u));
-            }
-        }
-    }
+    shade_path<MODE, WAVE_NEE>(sc, fp, blockIdx.x * blockDim.x + threadIdx.x, n, bounce, in, hitA, hitB, out, sh, sr,
+                               nee, Lpath, counters);
 }
 
-// Classification of traced shadow rays (the loop body of CalculateDirectLight,
-// ray_trace.comp.hybrid.hlsl:1167-1191): no hit before the light, or the light
-// itself -> unoccluded; glass -> step through (at most 4 queries per light);
-// anything else -> occluded, its contribution dropped.
 __global__ __launch_bounds__(kBlock) void k_shadow_resolve(SceneDev sc, uint32_t n, ShadowRays in,
                                                            const uint4 *__restrict__ hitA, ShadowRays out,
                                                            float4 *contrib, uint32_t *counter)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    bool more = false;
-    F3 no = f3(0, 0, 0), dir = f3(0, 0, 0), lp = f3(0, 0, 0);
-    float time = 0.0f;
-    uint32_t slot = 0, step = 0;
-    if (k < n) {
-        const float4 O = in.o[k], D = in.d[k], L = in.lp[k];
-        const uint4 H = hitA[k];
-        const F3 so = f3(O.x, O.y, O.z);
-        dir = f3(D.x, D.y, D.z);
-        lp = f3(L.x, L.y, L.z);
-        time = O.w;
-        slot = __float_as_uint(D.w);
-        step = __float_as_uint(L.w);
-        const float t = __uint_as_float(H.x);
-        const float bound = length(lp - so) - 0.001f;
-        if (H.y != ~0u && !(t >= bound)) {
-            const uint32_t ct = sc.objects[H.y].colorType;
-            if (ct != AVRT_DIFFUSE_LIGHT) {
-                if (ct != AVRT_DIELECTRIC) {
-                    contrib[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                } else if (step + 1u < 4u) {
-                    more = true;
-                    no = (so + dir * t) + (dir * 0.001f);
-                    step = step + 1u;
-                }
-            }
-        }
-    }
-    const uint32_t r = wave_append(more, counter);
-    if (more) {
-        out.o[r] = make_float4(no.x, no.y, no.z, time);
-        out.d[r] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(slot));
-        out.lp[r] = make_float4(lp.x, lp.y, lp.z, __uint_as_float(step));
-    }
+    shadow_resolve_ray(sc, blockIdx.x * blockDim.x + threadIdx.x, n, in, hitA, out, contrib, counter);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams fp, uint32_t first_sample, uint32_t ns,
+                                                    const float4 *Lpath, float4 *image, uchar4 *display)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fp.width * fp.height) return;
+    resolve_pixel<MODE>(fp, p, first_sample, ns, Lpath, image, display);
 }
 
 template <bool REF_ORDER>
@@ -362,52 +164,6 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev sc, FrameParams fp, 
     }
     const uint32_t tot = wave_sum(traced);
     if (__lane_id() == 0 && tot) atomicAdd(&counters[2], tot);
-}
-
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams fp, uint32_t first_sample, uint32_t ns,
-                                                    const float4 *Lpath, float4 *image, uchar4 *display)
-{
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t P = fp.width * fp.height;
-    if (p >= P) return;
-    float4 acc = image[p];
-    for (uint32_t s = 0; s < ns; s++) {
-        const uint32_t batch = fp.batch + first_sample + s;
-        const float4 c = Lpath[static_cast<size_t>(s) * P + p];
-        if (MODE == 0) {
-            const float px = batch > 0 ? acc.x : 0.0f;
-            const float py = batch > 0 ? acc.y : 0.0f;
-            const float pz = batch > 0 ? acc.z : 0.0f;
-            const float bf = static_cast<float>(batch), bn = static_cast<float>(batch + 1u);
-            acc.x = (px * bf + c.x) / bn;
-            acc.y = (py * bf + c.y) / bn;
-            acc.z = (pz * bf + c.z) / bn;
-        } else {
-            if (batch == 0) {
-                acc.x = c.x;
-                acc.y = c.y;
-                acc.z = c.z;
-            } else {
-                const float w = 1.0f / static_cast<float>(batch + 1u);
-                acc.x = acc.x + w * (c.x - acc.x);
-                acc.y = acc.y + w * (c.y - acc.y);
-                acc.z = acc.z + w * (c.z - acc.z);
-            }
-            if (fp.fp16) {
-                acc.x = round_half(acc.x);
-                acc.y = round_half(acc.y);
-                acc.z = round_half(acc.z);
-            }
-        }
-    }
-    acc.w = 1.0f;
-    image[p] = acc;
-    if (MODE == 1) {
-        const float e = fp.exposure;
-        display[p] = make_uchar4(unorm8(to_srgb(aces(acc.x * e))), unorm8(to_srgb(aces(acc.y * e))),
-                                 unorm8(to_srgb(aces(acc.z * e))), 255);
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_display_only(FrameParams fp, const float4 *image, uchar4 *display)

@@ -1,0 +1,647 @@
+// avrt_serve.hip - the served, batched form of the ray-tracing path.
+//
+// A server keeps scenes resident on the device (BVHs, objects, textures), takes
+// render jobs (scene, Config, mode) as they arrive, admits them while slots and
+// the HBM budget allow, and advances every admitted job in rounds. A round is
+// one batched wavefront over the rays of all admitted jobs:
+//
+//   raygen -> [extend -> shade -> shadow (trace, resolve through glass,
+//   finalize)] x depth -> resolve
+//
+// with ONE launch per stage for all jobs together. Each job owns a contiguous,
+// 256-aligned region of the path, shadow-ray and NEE queues and its own queue
+// counters; a workgroup serves exactly one job (block -> job by a binary search
+// of the launch table passed as a kernel argument), so the job's scene and
+// frame constants are uniform per wave and live in scalar registers, and the
+// stage code is the plain renderer's (the shared stage bodies of
+// avrt_shade.h). Jobs with different scenes, resolutions, sample counts, depths
+// and integrators share every launch.
+//
+// Determinism: a job's paths use the plain renderer's RNG streams (pixel,
+// sample frame) and its samples are folded into its accumulation image in
+// frame order, in rounds of consecutive samples, so a job renders the same
+// image alone, in any mix, and through avrt_render.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/avrt.h"
+#include "avrt_ctx.h"
+#include "avrt_device.h"
+#include "avrt_internal.h"
+#include "avrt_shade.h"
+
+using namespace avrt;
+using namespace avrt::dev;
+
+namespace {
+
+constexpr uint32_t kMaxRoundJobs = 64;
+
+// One launch over the jobs of a round: job j covers blocks
+// [blk[j], blk[j+1]) and `count[j]` queue entries.
+struct SegLaunch {
+    uint32_t njobs;
+    uint32_t blk[kMaxRoundJobs + 1];
+    uint32_t count[kMaxRoundJobs];
+};
+
+// Per round and job, on the device.
+struct JobDev {
+    SceneDev sc;
+    FrameParams fp;
+    uint32_t mode;          // AVRT_MODE_*
+    uint32_t path_base;     // first path slot (queue region, NEE records: path_base; shadow rays: 4 * path_base)
+    uint32_t first_sample;  // first sample frame of this round
+    uint32_t ns;            // sample frames this round
+    float4 *image;
+    uchar4 *display;
+};
+
+__device__ __forceinline__ uint32_t job_of(const SegLaunch &L, uint32_t block)
+{
+    uint32_t lo = 0, hi = L.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.blk[mid] <= block) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ PathBufs at(PathBufs b, uint32_t off) { return PathBufs{b.o + off, b.d + off, b.t + off}; }
+__device__ __forceinline__ ShadowRays at(ShadowRays b, uint32_t off) { return ShadowRays{b.o + off, b.d + off, b.lp + off}; }
+
+__global__ __launch_bounds__(kBlock) void k_seg_raygen(SegLaunch L, const JobDev *__restrict__ jobs, PathBufs out,
+                                                       float4 *Lpath)
+{
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    if (local >= L.count[j]) return;
+    const JobDev &J = jobs[j];
+    raygen_path(J.fp, J.first_sample, local, J.path_base + local, out, Lpath);
+}
+
+// Closest-hit queries of the jobs' queues (extension rays, or shadow rays
+// bounded by their light points).
+template <bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_seg_trace(SegLaunch L, const JobDev *__restrict__ jobs,
+                                                      const float4 *__restrict__ ro, const float4 *__restrict__ rd,
+                                                      const float4 *__restrict__ rlp, uint4 *__restrict__ hitA,
+                                                      float *__restrict__ hitB)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    if (local >= L.count[j]) return;
+    const SceneDev sc = jobs[j].sc;
+    const uint32_t idx = (SHADOW ? 4u : 1u) * jobs[j].path_base + local;
+    uint32_t spill_mem[kStackSpill];
+    const float4 O = ro[idx];
+    const float4 D = rd[idx];
+    const F3 o = f3(O.x, O.y, O.z);
+    float far = 1000000.0f;
+    if (SHADOW) {
+        const float4 Lp = rlp[idx];
+        far = length(f3(Lp.x, Lp.y, Lp.z) - o) - 0.001f;
+    }
+    Traversal<false, kStackLds, kBlock, kStackSpill> tr;
+    if (tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem)) {
+        while (tr.step(sc)) {
+        }
+    }
+    hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+    hitB[idx] = tr.hit.beta;
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev *__restrict__ jobs, uint32_t bounce,
+                                                      PathBufs in, const uint4 *__restrict__ hitA,
+                                                      const float *__restrict__ hitB, PathBufs out, ShadowRays sr,
+                                                      NeeBufs nee, float4 *Lpath, uint32_t *counters)
+{
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    const SceneDev sc = jobs[j].sc;
+    const FrameParams fp = jobs[j].fp;
+    const uint32_t b = jobs[j].path_base;
+    const ShadowBufs none{nullptr, nullptr, nullptr, nullptr};
+    const NeeBufs nj{nee.tp + b, nee.contrib + 4u * b};
+    if (jobs[j].mode == AVRT_MODE_PATH_TRACE)
+        shade_path<0, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * j);
+    else
+        shade_path<1, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * j);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
+                                                               ShadowRays in, const uint4 *__restrict__ hitA,
+                                                               ShadowRays out, float4 *contrib, uint32_t *counters)
+{
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    const uint32_t s = 4u * jobs[j].path_base;
+    const SceneDev sc = jobs[j].sc;
+    shadow_resolve_ray(sc, local, L.count[j], at(in, s), hitA + s, at(out, s), contrib + s, counters + 4u * j + 3u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_nee_finalize(SegLaunch L, const JobDev *__restrict__ jobs, NeeBufs nee,
+                                                             float4 *Lpath)
+{
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t r = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    if (r >= L.count[j]) return;
+    const JobDev &J = jobs[j];
+    const uint32_t nl = J.fp.num_lights;
+    const float4 *contrib = nee.contrib + 4u * J.path_base;
+    F3 sum = f3(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < nl; i++) {
+        const float4 c = contrib[r * nl + i];
+        sum = sum + f3(c.x, c.y, c.z);
+    }
+    const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
+    if (luma > 8.0f) sum = sum * (8.0f / luma);
+    const float4 T = nee.tp[J.path_base + r];
+    add_radiance(Lpath, __float_as_uint(T.w), f3(T.x, T.y, T.z) * sum);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
+                                                        const float4 *Lpath)
+{
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t p = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    if (p >= L.count[j]) return;
+    const JobDev &J = jobs[j];
+    if (J.mode == AVRT_MODE_PATH_TRACE)
+        resolve_pixel<0>(J.fp, p, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+    else
+        resolve_pixel<1>(J.fp, p, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct ServeScene {
+    avrt_ctx *ctx = nullptr; // holds the scene's device buffers
+    size_t bytes = 0;
+    uint32_t refs = 0;       // admitted, unreleased jobs using it
+};
+
+struct ServeJob {
+    uint64_t id = 0;
+    uint32_t scene = 0;
+    int mode = 0;
+    AvrtConfig cfg{};
+    FrameParams fp{};
+    uint32_t P = 0, spp = 0, done = 0;
+    int state = AVRT_JOB_QUEUED;
+    size_t bytes = 0;
+    avrt::DevBuf image, display;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+struct avrt_server {
+    AvrtServerOptions opts{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<ServeScene> scenes;
+    std::unordered_map<uint64_t, ServeJob> jobs;
+    std::deque<uint64_t> queue;    // submitted, waiting for admission
+    std::vector<uint64_t> active;  // admitted, in admission order
+    uint64_t next_id = 1;
+    size_t used = 0;               // HBM charged: scenes, job targets, round buffers
+    size_t round_cap = 0;
+    avrt::DevBuf po[2], pd[2], pt[2], hitA, hitB, Lpath, sro[2], srd[2], srl[2], nee_tp, nee_contrib, counters, jobtab;
+    AvrtServerStats stats{};
+    std::chrono::steady_clock::time_point t0;
+};
+
+namespace {
+
+double now_s(const avrt_server *s)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+int serr(avrt_server *s, int code, const std::string &m)
+{
+    s->err = m;
+    return code;
+}
+
+#define SHIPX(srv, call)                                                                                        \
+    do {                                                                                                        \
+        hipError_t e_ = (call);                                                                                 \
+        if (e_ != hipSuccess) return serr((srv), AVRT_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int salloc(avrt_server *s, avrt::DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return AVRT_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    s->used -= std::min(s->used, b.bytes);
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        return serr(s, AVRT_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    b.bytes = bytes;
+    s->used += bytes;
+    return AVRT_OK;
+}
+
+void sfree(avrt_server *s, avrt::DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    s->used -= std::min(s->used, b.bytes);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+size_t ctx_bytes(const avrt_ctx *c)
+{
+    const avrt::DevBuf *bufs[] = {&c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects,
+                                  &c->d_geom,  &c->d_tlas2,   &c->d_lights, &c->d_instances, &c->d_tex};
+    size_t n = 0;
+    for (const avrt::DevBuf *b : bufs) n += b->bytes;
+    return n;
+}
+
+inline uint32_t align256(uint32_t n) { return (n + 255u) & ~255u; }
+
+int ensure_round(avrt_server *s, size_t paths)
+{
+    if (paths <= s->round_cap) return AVRT_OK;
+    int rc = 0;
+    for (int i = 0; i < 2; i++) {
+        if ((rc = salloc(s, s->po[i], paths * 16)) || (rc = salloc(s, s->pd[i], paths * 16)) ||
+            (rc = salloc(s, s->pt[i], paths * 16)))
+            return rc;
+        if ((rc = salloc(s, s->sro[i], 4 * paths * 16)) || (rc = salloc(s, s->srd[i], 4 * paths * 16)) ||
+            (rc = salloc(s, s->srl[i], 4 * paths * 16)))
+            return rc;
+    }
+    if ((rc = salloc(s, s->hitA, 4 * paths * 16)) || (rc = salloc(s, s->hitB, 4 * paths * 4)) ||
+        (rc = salloc(s, s->Lpath, paths * 16)) || (rc = salloc(s, s->nee_tp, paths * 16)) ||
+        (rc = salloc(s, s->nee_contrib, 4 * paths * 16)) || (rc = salloc(s, s->counters, kMaxRoundJobs * 16)) ||
+        (rc = salloc(s, s->jobtab, kMaxRoundJobs * sizeof(JobDev))))
+        return rc;
+    s->round_cap = paths;
+    return AVRT_OK;
+}
+
+// Launch table over `n` jobs with `count` entries each; returns the grid.
+uint32_t make_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
+{
+    L.njobs = n;
+    uint32_t b = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        L.blk[j] = b;
+        L.count[j] = count[j];
+        b += (count[j] + kBlock - 1) / kBlock;
+    }
+    L.blk[n] = b;
+    return b;
+}
+
+int read_job_counters(avrt_server *s, uint32_t n, uint32_t *host)
+{
+    SHIPX(s, hipMemcpyAsync(host, s->counters.p, 16 * n, hipMemcpyDeviceToHost, s->stream));
+    SHIPX(s, hipStreamSynchronize(s->stream));
+    return AVRT_OK;
+}
+
+// Admission: jobs in arrival order while slots and the HBM budget allow.
+int admit(avrt_server *s)
+{
+    while (!s->queue.empty() && s->active.size() < s->opts.max_jobs) {
+        ServeJob &j = s->jobs[s->queue.front()];
+        const size_t need = static_cast<size_t>(j.P) * 20;
+        if (s->used + need > s->opts.hbm_budget_bytes) break;
+        int rc = salloc(s, j.image, static_cast<size_t>(j.P) * 16);
+        if (!rc) rc = salloc(s, j.display, static_cast<size_t>(j.P) * 4);
+        if (rc) return rc;
+        SHIPX(s, hipMemsetAsync(j.image.p, 0, static_cast<size_t>(j.P) * 16, s->stream));
+        SHIPX(s, hipMemsetAsync(j.display.p, 0, static_cast<size_t>(j.P) * 4, s->stream));
+        j.state = AVRT_JOB_ACTIVE;
+        j.t_admit = now_s(s);
+        s->scenes[j.scene].refs++;
+        s->active.push_back(j.id);
+        s->queue.pop_front();
+    }
+    return AVRT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int avrt_server_create(const AvrtServerOptions *opts, avrt_server **out)
+{
+    if (out == nullptr) return AVRT_E_INVALID_ARG;
+    *out = nullptr;
+    AvrtServerOptions o{};
+    if (opts) o = *opts;
+    if (o.hbm_budget_bytes == 0) o.hbm_budget_bytes = 200ull << 30;
+    if (o.max_jobs == 0 || o.max_jobs > kMaxRoundJobs) o.max_jobs = kMaxRoundJobs;
+    if (o.round_paths == 0) o.round_paths = 1u << 24;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return AVRT_E_DEVICE;
+    if (o.device < 0 || o.device >= ndev) return AVRT_E_INVALID_ARG;
+    if (hipSetDevice(o.device) != hipSuccess) return AVRT_E_DEVICE;
+    avrt_server *s = new avrt_server();
+    s->opts = o;
+    s->t0 = std::chrono::steady_clock::now();
+    float lut[256];
+    for (int i = 0; i < 256; i++) {
+        const double v = i / 255.0;
+        lut[i] = static_cast<float>(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess) {
+        avrt_server_destroy(s);
+        return AVRT_E_DEVICE;
+    }
+    // the round queues are sized once, for round_paths paths, and charged to
+    // the budget before any scene or job
+    const int rc = ensure_round(s, o.round_paths);
+    if (rc || s->used > o.hbm_budget_bytes) {
+        avrt_server_destroy(s);
+        return rc ? rc : AVRT_E_CAPACITY;
+    }
+    *out = s;
+    return AVRT_OK;
+}
+
+int avrt_server_destroy(avrt_server *s)
+{
+    if (s == nullptr) return AVRT_E_INVALID_ARG;
+    (void)hipSetDevice(s->opts.device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &kv : s->jobs) {
+        sfree(s, kv.second.image);
+        sfree(s, kv.second.display);
+    }
+    avrt::DevBuf *bufs[] = {&s->po[0],  &s->po[1],  &s->pd[0],  &s->pd[1],  &s->pt[0],  &s->pt[1],   &s->hitA,
+                            &s->hitB,   &s->Lpath,  &s->sro[0], &s->sro[1], &s->srd[0], &s->srd[1],  &s->srl[0],
+                            &s->srl[1], &s->nee_tp, &s->nee_contrib, &s->counters, &s->jobtab};
+    for (avrt::DevBuf *b : bufs) sfree(s, *b);
+    for (ServeScene &sc : s->scenes)
+        if (sc.ctx) avrt_destroy(sc.ctx);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return AVRT_OK;
+}
+
+const char *avrt_server_last_error(const avrt_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int avrt_server_add_scene(avrt_server *s, avrt_ctx *scene, uint32_t *scene_id)
+{
+    if (s == nullptr || scene == nullptr || scene_id == nullptr) return AVRT_E_INVALID_ARG;
+    if (!scene->scene_loaded) return serr(s, AVRT_E_STATE, "the context holds no uploaded scene");
+    if (scene->device != s->opts.device) return serr(s, AVRT_E_INVALID_ARG, "scene on another device");
+    ServeScene sc;
+    sc.ctx = scene;
+    sc.bytes = ctx_bytes(scene);
+    if (s->used + sc.bytes > s->opts.hbm_budget_bytes) return serr(s, AVRT_E_CAPACITY, "scene exceeds the HBM budget");
+    s->used += sc.bytes;
+    s->scenes.push_back(sc);
+    *scene_id = static_cast<uint32_t>(s->scenes.size() - 1);
+    return AVRT_OK;
+}
+
+int avrt_server_submit(avrt_server *s, uint32_t scene_id, const AvrtConfig *cfg, int mode, uint64_t *job_id)
+{
+    if (s == nullptr || cfg == nullptr || job_id == nullptr) return AVRT_E_INVALID_ARG;
+    if (scene_id >= s->scenes.size()) return serr(s, AVRT_E_INVALID_ARG, "unknown scene");
+    if (mode != AVRT_MODE_PATH_TRACE && mode != AVRT_MODE_RAY_TRACE) return serr(s, AVRT_E_INVALID_ARG, "unknown mode");
+    avrt_ctx *c = s->scenes[scene_id].ctx;
+    if (cfg->width == 0 || cfg->height == 0) return serr(s, AVRT_E_INVALID_ARG, "zero image size");
+    const uint64_t P = static_cast<uint64_t>(cfg->width) * cfg->height;
+    if (P > s->opts.round_paths) return serr(s, AVRT_E_CAPACITY, "one sample frame exceeds the round capacity");
+    if (mode == AVRT_MODE_PATH_TRACE && c->has_tri_or_mesh)
+        return serr(s, AVRT_E_UNSUPPORTED, "path_trace mode has no triangle or mesh intersector");
+    if (mode == AVRT_MODE_RAY_TRACE && c->nlights > kWaveLights)
+        return serr(s, AVRT_E_UNSUPPORTED, "served jobs take at most 4 lights");
+    ServeJob j;
+    j.id = s->next_id++;
+    j.scene = scene_id;
+    j.mode = mode;
+    j.cfg = *cfg;
+    frame_params(c, cfg, j.fp);
+    j.P = static_cast<uint32_t>(P);
+    j.spp = cfg->samples ? cfg->samples : 1u;
+    j.t_submit = now_s(s);
+    s->queue.push_back(j.id);
+    *job_id = j.id;
+    s->jobs.emplace(j.id, std::move(j));
+    s->stats.submitted++;
+    return AVRT_OK;
+}
+
+int avrt_server_step(avrt_server *s, uint32_t *completed)
+{
+    if (s == nullptr) return AVRT_E_INVALID_ARG;
+    if (completed) *completed = 0;
+    SHIPX(s, hipSetDevice(s->opts.device));
+    int rc = admit(s);
+    if (rc) return rc;
+    if (s->active.empty()) return AVRT_OK;
+    // Round composition: active jobs in admission order, each taking as many
+    // of its remaining sample frames as the round capacity leaves.
+    std::vector<ServeJob *> rj;
+    std::vector<JobDev> jd;
+    std::vector<uint32_t> npaths;
+    uint32_t base = 0;
+    const uint32_t cap = s->opts.round_paths;
+    for (uint64_t id : s->active) {
+        if (rj.size() == kMaxRoundJobs) break;
+        ServeJob &j = s->jobs[id];
+        const uint32_t left = cap - base;
+        if (left < j.P) continue;
+        const uint32_t ns = std::min(j.spp - j.done, left / j.P);
+        const uint32_t n = j.P * ns;
+        JobDev d{};
+        d.sc = scene_dev(s->scenes[j.scene].ctx);
+        d.fp = j.fp;
+        d.mode = static_cast<uint32_t>(j.mode);
+        d.path_base = base;
+        d.first_sample = j.done;
+        d.ns = ns;
+        d.image = j.image.as<float4>();
+        d.display = j.display.as<uchar4>();
+        rj.push_back(&j);
+        jd.push_back(d);
+        npaths.push_back(n);
+        base += align256(n);
+    }
+    if (rj.empty()) return AVRT_OK;
+    const uint32_t J = static_cast<uint32_t>(rj.size());
+    if ((rc = ensure_round(s, base))) return rc;
+    SHIPX(s, hipMemcpyAsync(s->jobtab.p, jd.data(), J * sizeof(JobDev), hipMemcpyHostToDevice, s->stream));
+    const JobDev *jobs = s->jobtab.as<const JobDev>();
+    PathBufs buf[2];
+    for (int i = 0; i < 2; i++) buf[i] = PathBufs{s->po[i].as<float4>(), s->pd[i].as<float4>(), s->pt[i].as<float4>()};
+    ShadowRays sr[2];
+    for (int i = 0; i < 2; i++) sr[i] = ShadowRays{s->sro[i].as<float4>(), s->srd[i].as<float4>(), s->srl[i].as<float4>()};
+    NeeBufs nee{s->nee_tp.as<float4>(), s->nee_contrib.as<float4>()};
+    float4 *Lpath = s->Lpath.as<float4>();
+    uint32_t *ctr = s->counters.as<uint32_t>();
+    SegLaunch L;
+    uint32_t grid = make_launch(L, npaths.data(), J);
+    hipLaunchKernelGGL(k_seg_raygen, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[0], Lpath);
+    SHIPX(s, hipGetLastError());
+    std::vector<uint32_t> cnt(npaths), hc(4 * J), nsr(J), nrec(J);
+    uint32_t maxdepth = 0;
+    for (const JobDev &d : jd) maxdepth = std::max(maxdepth, d.fp.depth);
+    int cur = 0;
+    for (uint32_t b = 0; b < maxdepth; b++) {
+        uint64_t tot = 0;
+        for (uint32_t j = 0; j < J; j++) tot += cnt[j];
+        if (tot == 0) break;
+        if (b == 0) s->stats.rays_primary += tot;
+        else s->stats.rays_bounce += tot;
+        grid = make_launch(L, cnt.data(), J);
+        hipLaunchKernelGGL((k_seg_trace<false>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[cur].o, buf[cur].d,
+                           static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>());
+        SHIPX(s, hipGetLastError());
+        SHIPX(s, hipMemsetAsync(ctr, 0, 16 * J, s->stream));
+        hipLaunchKernelGGL(k_seg_shade, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, b, buf[cur],
+                           s->hitA.as<const uint4>(), s->hitB.as<const float>(), buf[cur ^ 1], sr[0], nee, Lpath, ctr);
+        SHIPX(s, hipGetLastError());
+        if ((rc = read_job_counters(s, J, hc.data()))) return rc;
+        uint64_t any_rec = 0;
+        for (uint32_t j = 0; j < J; j++) {
+            cnt[j] = hc[4 * j];
+            nrec[j] = hc[4 * j + 1];
+            nsr[j] = hc[4 * j + 2];
+            any_rec += nrec[j];
+        }
+        if (any_rec) {
+            int q = 0;
+            for (int step = 0; step < 4; step++) {
+                uint64_t tsr = 0;
+                for (uint32_t j = 0; j < J; j++) tsr += nsr[j];
+                if (tsr == 0) break;
+                s->stats.rays_shadow += tsr;
+                grid = make_launch(L, nsr.data(), J);
+                hipLaunchKernelGGL((k_seg_trace<true>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q].o, sr[q].d,
+                                   sr[q].lp, s->hitA.as<uint4>(), s->hitB.as<float>());
+                SHIPX(s, hipGetLastError());
+                SHIPX(s, hipMemsetAsync(ctr, 0, 16 * J, s->stream));
+                hipLaunchKernelGGL(k_seg_shadow_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q],
+                                   s->hitA.as<const uint4>(), sr[q ^ 1], nee.contrib, ctr);
+                SHIPX(s, hipGetLastError());
+                if ((rc = read_job_counters(s, J, hc.data()))) return rc;
+                for (uint32_t j = 0; j < J; j++) nsr[j] = hc[4 * j + 3];
+                q ^= 1;
+            }
+            grid = make_launch(L, nrec.data(), J);
+            hipLaunchKernelGGL(k_seg_nee_finalize, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, nee, Lpath);
+            SHIPX(s, hipGetLastError());
+        }
+        cur ^= 1;
+    }
+    std::vector<uint32_t> pix(J);
+    for (uint32_t j = 0; j < J; j++) pix[j] = rj[j]->P;
+    grid = make_launch(L, pix.data(), J);
+    hipLaunchKernelGGL(k_seg_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, Lpath);
+    SHIPX(s, hipGetLastError());
+    SHIPX(s, hipStreamSynchronize(s->stream));
+    s->stats.rounds++;
+    const double t = now_s(s);
+    for (uint32_t j = 0; j < J; j++) {
+        ServeJob &job = *rj[j];
+        job.done += jd[j].ns;
+        s->stats.paths += npaths[j];
+        if (job.done >= job.spp) {
+            job.state = AVRT_JOB_DONE;
+            job.t_done = t;
+            s->stats.completed++;
+            if (completed) (*completed)++;
+        }
+    }
+    std::vector<uint64_t> still;
+    for (uint64_t id : s->active)
+        if (s->jobs[id].state == AVRT_JOB_ACTIVE) still.push_back(id);
+    s->active.swap(still);
+    return admit(s);
+}
+
+int avrt_server_job_status(const avrt_server *s, uint64_t job, AvrtJobStatus *out)
+{
+    if (s == nullptr || out == nullptr) return AVRT_E_INVALID_ARG;
+    auto it = s->jobs.find(job);
+    if (it == s->jobs.end()) return AVRT_E_INVALID_ARG;
+    const ServeJob &j = it->second;
+    out->state = j.state;
+    out->samples_done = j.done;
+    out->samples = j.spp;
+    out->t_submit = j.t_submit;
+    out->t_admit = j.t_admit;
+    out->t_done = j.t_done;
+    return AVRT_OK;
+}
+
+int avrt_server_read_job(avrt_server *s, uint64_t job, float *image, size_t image_bytes, uint8_t *display,
+                         size_t display_bytes)
+{
+    if (s == nullptr) return AVRT_E_INVALID_ARG;
+    auto it = s->jobs.find(job);
+    if (it == s->jobs.end()) return serr(s, AVRT_E_INVALID_ARG, "unknown job");
+    ServeJob &j = it->second;
+    if (j.state != AVRT_JOB_DONE) return serr(s, AVRT_E_STATE, "job not finished");
+    SHIPX(s, hipSetDevice(s->opts.device));
+    if (image) {
+        if (image_bytes < static_cast<size_t>(j.P) * 16) return serr(s, AVRT_E_CAPACITY, "image destination too small");
+        SHIPX(s, hipMemcpyAsync(image, j.image.p, static_cast<size_t>(j.P) * 16, hipMemcpyDeviceToHost, s->stream));
+    }
+    if (display) {
+        if (display_bytes < static_cast<size_t>(j.P) * 4) return serr(s, AVRT_E_CAPACITY, "display destination too small");
+        SHIPX(s, hipMemcpyAsync(display, j.display.p, static_cast<size_t>(j.P) * 4, hipMemcpyDeviceToHost, s->stream));
+    }
+    SHIPX(s, hipStreamSynchronize(s->stream));
+    return AVRT_OK;
+}
+
+int avrt_server_release_job(avrt_server *s, uint64_t job)
+{
+    if (s == nullptr) return AVRT_E_INVALID_ARG;
+    auto it = s->jobs.find(job);
+    if (it == s->jobs.end()) return serr(s, AVRT_E_INVALID_ARG, "unknown job");
+    ServeJob &j = it->second;
+    if (j.state == AVRT_JOB_ACTIVE) return serr(s, AVRT_E_STATE, "job still running");
+    if (j.state == AVRT_JOB_QUEUED) {
+        s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), job), s->queue.end());
+    } else {
+        s->scenes[j.scene].refs--;
+    }
+    sfree(s, j.image);
+    sfree(s, j.display);
+    s->jobs.erase(it);
+    return AVRT_OK;
+}
+
+int avrt_server_get_stats(const avrt_server *s, AvrtServerStats *out)
+{
+    if (s == nullptr || out == nullptr) return AVRT_E_INVALID_ARG;
+    *out = s->stats;
+    out->hbm_used_bytes = s->used;
+    out->queued = static_cast<uint32_t>(s->queue.size());
+    out->active = static_cast<uint32_t>(s->active.size());
+    return AVRT_OK;
+}
+
+} // extern "C"

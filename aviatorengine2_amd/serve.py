@@ -1,0 +1,179 @@
+"""Serving: a request loop over avrt_server (include/avrt.h).
+
+    srv = Server(hbm_budget_gib=200)
+    sid = srv.add_scene(packed)                     # resident BVHs, objects, textures
+    job = srv.submit(sid, config, mode)             # queued; admitted by budget
+    srv.step()                                      # one batched round over all admitted jobs
+    srv.status(job), srv.read(job), srv.release(job)
+
+run_stream() drives a Poisson stream of jobs in real time: jobs are submitted
+when their arrival time has passed, every round advances all admitted jobs
+together, and each job's latency is measured from its scheduled arrival to the
+end of the round that finished it.
+
+The mixed stream (mixed_scenes / poisson_jobs) is drawn from 8 seeded scenes
+shaped like the sphere field (path_trace) and the icosphere triangle scene
+(ray_trace): resolution uniform over {256^2, 640x360, 1280x720, 1920x1080},
+samples log-uniform in [1, 64], depth 4..8.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import abi, scenes
+from .abi import AvrtError, AvrtJobStatus, AvrtServerOptions, AvrtServerStats, ptr
+from .renderer import Renderer
+
+
+class Server:
+    def __init__(self, device=0, hbm_budget_gib=200.0, max_jobs=64, round_paths=1 << 24):
+        self.lib = abi.load_library()
+        o = AvrtServerOptions()
+        o.device = device
+        o.max_jobs = max_jobs
+        o.hbm_budget_bytes = int(hbm_budget_gib * (1 << 30))
+        o.round_paths = round_paths
+        h = ctypes.c_void_p()
+        rc = self.lib.avrt_server_create(ctypes.byref(o), ctypes.byref(h))
+        if rc:
+            raise AvrtError(rc, "avrt_server_create")
+        self.h = h
+        self.jobs = {}
+
+    def close(self):
+        if self.h is not None:
+            self.lib.avrt_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise AvrtError(rc, f"{what}: {self.lib.avrt_server_last_error(self.h).decode()}")
+
+    def add_scene(self, packed) -> int:
+        """Upload a packed scene into a context and hand it to the server."""
+        r = Renderer().Initialize()
+        r.LoadScene(packed)
+        sid = ctypes.c_uint32()
+        self._check(self.lib.avrt_server_add_scene(self.h, r.ctx, ctypes.byref(sid)), "avrt_server_add_scene")
+        r.ctx = None  # owned by the server now
+        return sid.value
+
+    def submit(self, scene_id, config, mode) -> int:
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        jid = ctypes.c_uint64()
+        self._check(self.lib.avrt_server_submit(self.h, scene_id, c.ctypes.data_as(ctypes.c_void_p), int(mode),
+                                                ctypes.byref(jid)), "avrt_server_submit")
+        self.jobs[jid.value] = (int(c["width"][0]), int(c["height"][0]))
+        return jid.value
+
+    def step(self) -> int:
+        n = ctypes.c_uint32()
+        self._check(self.lib.avrt_server_step(self.h, ctypes.byref(n)), "avrt_server_step")
+        return n.value
+
+    def status(self, job) -> AvrtJobStatus:
+        s = AvrtJobStatus()
+        self._check(self.lib.avrt_server_job_status(self.h, job, ctypes.byref(s)), "avrt_server_job_status")
+        return s
+
+    def read(self, job):
+        w, h = self.jobs[job]
+        img = np.empty((h, w, 4), np.float32)
+        disp = np.empty((h, w, 4), np.uint8)
+        self._check(self.lib.avrt_server_read_job(self.h, job, ptr(img), img.nbytes, ptr(disp), disp.nbytes),
+                    "avrt_server_read_job")
+        return img, disp
+
+    def release(self, job):
+        self._check(self.lib.avrt_server_release_job(self.h, job), "avrt_server_release_job")
+        self.jobs.pop(job, None)
+
+    def stats(self) -> dict:
+        s = AvrtServerStats()
+        self._check(self.lib.avrt_server_get_stats(self.h, ctypes.byref(s)), "avrt_server_get_stats")
+        return s.as_dict()
+
+
+RESOLUTIONS = [(256, 256), (640, 360), (1280, 720), (1920, 1080)]
+
+
+def mixed_scenes(n=8, seed=0x5E12FE0D):
+    """n seeded scenes: even ones sphere fields (path_trace), odd ones
+    noise-displaced icosphere triangle scenes (ray_trace)."""
+    rng = scenes.Pcg32(seed)
+    out = []
+    for k in range(n):
+        s = int(rng.next_u32())
+        if k % 2 == 0:
+            spec = scenes.sphere_field(seed=s)
+            out.append((scenes.pack(spec), abi.MODE_PATH_TRACE))
+        else:
+            spec = scenes.icosphere_tris(seed=s, level=5 + (k // 2) % 2)
+            out.append((scenes.pack(spec), abi.MODE_RAY_TRACE))
+    return out
+
+
+def poisson_jobs(n_jobs, rate_hz, n_scenes=8, seed=0x10B5, max_spp=64):
+    """(arrival_s, scene, width, height, spp, depth) per job."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_hz, n_jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    jobs = []
+    for i in range(n_jobs):
+        w, h = RESOLUTIONS[int(rng.integers(0, len(RESOLUTIONS)))]
+        spp = int(min(max_spp, max(1, math.floor(math.exp(rng.uniform(0.0, math.log(max_spp + 1)))))))
+        depth = int(rng.integers(4, 9))
+        jobs.append((float(arrivals[i]), int(rng.integers(0, n_scenes)), w, h, spp, depth))
+    return jobs
+
+
+def job_config(packed, width, height, spp, depth, batch=0):
+    c = packed.config.copy()
+    c["width"], c["height"], c["samples"], c["depth"], c["batch"] = width, height, spp, depth, batch
+    return c
+
+
+def run_stream(srv, scene_ids, scene_list, jobs, read_back=False, progress=None):
+    """Drive the stream in real time. Returns per-job latencies (s), the wall
+    time, the server stats and (read_back) the images."""
+    t0 = time.perf_counter()
+    pending = list(enumerate(jobs))
+    live = {}
+    lat = np.full(len(jobs), np.nan)
+    images = {}
+    next_log = t0 + 30.0
+    while pending or live:
+        now = time.perf_counter() - t0
+        while pending and pending[0][1][0] <= now:
+            i, (arr, sc, w, h, spp, depth) = pending.pop(0)
+            pk, mode = scene_list[sc]
+            jid = srv.submit(scene_ids[sc], job_config(pk, w, h, spp, depth), mode)
+            live[jid] = (i, arr)
+        if not live:
+            if pending:
+                time.sleep(max(0.0, min(pending[0][1][0] - now, 0.05)))
+            continue
+        srv.step()
+        t_end = time.perf_counter() - t0
+        for jid in list(live):
+            if srv.status(jid).state == abi.JOB_DONE:
+                i, arr = live.pop(jid)
+                lat[i] = t_end - arr
+                if read_back:
+                    images[i] = srv.read(jid)
+                srv.release(jid)
+        if progress and time.perf_counter() > next_log:
+            next_log += 30.0
+            progress(f"{int(np.isfinite(lat).sum())}/{len(jobs)} jobs done")
+    wall = time.perf_counter() - t0
+    return lat, wall, srv.stats(), images

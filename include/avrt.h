@@ -380,6 +380,68 @@ int avrt_update_vertices(avrt_ctx *ctx, const AvrtVertex *vertices,
 int avrt_read_blas(avrt_ctx *ctx, AvrtBVHNode *dst, uint32_t count);
 
 /* Library version string. */
+/* ------------------------------------------------------------------------- */
+/* Serving: render jobs of resident scenes, batched across jobs.             */
+/* ------------------------------------------------------------------------- */
+/* No counterpart in the reference, which renders one scene per process
+ * (renderer.cpp:1330-1583). A server keeps scenes resident, admits jobs in
+ * arrival order while job slots and the HBM budget allow, and advances all
+ * admitted jobs together in rounds: every stage of a round (raygen, extend,
+ * shade, shadow trace / glass steps / NEE sum, resolve) is ONE kernel launch
+ * over the rays of all jobs, each job holding its own queue region and
+ * counters. A job's image is the one avrt_render gives for the same scene,
+ * Config and mode (same RNG streams, samples folded in frame order): alone or
+ * in any mix. Ray-trace jobs take at most 4 lights. */
+typedef struct avrt_server avrt_server;
+
+typedef struct AvrtServerOptions {
+    int device;
+    uint32_t max_jobs;          /* admitted jobs at once (<= 64; 0 = 64)        */
+    uint64_t hbm_budget_bytes;  /* scenes + job targets + round queues (0 = 200 GiB) */
+    uint32_t round_paths;       /* paths per round (0 = 2^24); one sample frame of
+                                   every job must fit. The round queues (about
+                                   660 bytes per path) are allocated at creation
+                                   and charged to the budget first.             */
+    uint32_t reserved[3];
+} AvrtServerOptions;
+
+#define AVRT_JOB_QUEUED 0
+#define AVRT_JOB_ACTIVE 1
+#define AVRT_JOB_DONE 2
+
+typedef struct AvrtJobStatus {
+    int state;                  /* AVRT_JOB_*                                   */
+    uint32_t samples_done, samples;
+    double t_submit, t_admit, t_done; /* seconds since the server was created  */
+} AvrtJobStatus;
+
+typedef struct AvrtServerStats {
+    uint64_t submitted, completed, rounds;
+    uint64_t rays_primary, rays_bounce, rays_shadow, paths;
+    uint64_t hbm_used_bytes;
+    uint32_t queued, active;
+} AvrtServerStats;
+
+int avrt_server_create(const AvrtServerOptions *opts, avrt_server **out);
+int avrt_server_destroy(avrt_server *srv);
+const char *avrt_server_last_error(const avrt_server *srv);
+/* Takes ownership of a context loaded with avrt_upload_geometry / _textures /
+ * _scene: its device buffers become the resident scene `*scene_id`. */
+int avrt_server_add_scene(avrt_server *srv, avrt_ctx *scene, uint32_t *scene_id);
+/* Queue a job: config.samples sample frames from config.batch, mode as
+ * avrt_render. Admission happens in avrt_server_step. */
+int avrt_server_submit(avrt_server *srv, uint32_t scene_id, const AvrtConfig *config, int mode, uint64_t *job_id);
+/* Admit what fits, run one round over the admitted jobs, admit again.
+ * *completed = jobs finished by this round. */
+int avrt_server_step(avrt_server *srv, uint32_t *completed);
+int avrt_server_job_status(const avrt_server *srv, uint64_t job_id, AvrtJobStatus *out);
+/* Accumulation (RGBA32F) and display (RGBA8) of a finished job. */
+int avrt_server_read_job(avrt_server *srv, uint64_t job_id, float *image, size_t image_bytes,
+                         uint8_t *display, size_t display_bytes);
+/* Free a finished (or still queued) job's targets. */
+int avrt_server_release_job(avrt_server *srv, uint64_t job_id);
+int avrt_server_get_stats(const avrt_server *srv, AvrtServerStats *out);
+
 const char *avrt_version(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,122 @@
+"""Serving and batching on the GPU (avrt_server_*).
+
+The determinism bar: a job's accumulation image is bit-identical to the plain
+renderer's (avrt_render) for the same scene, Config and mode, whether the job
+runs alone or batched with other jobs of other scenes, resolutions, sample
+counts, depths and integrators, and however its samples are split across
+rounds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from aviatorengine2_amd import abi, scenes, serve
+from aviatorengine2_amd.renderer import Renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def scene_set():
+    sf = scenes.pack(scenes.sphere_field(seed=11))
+    ico = scenes.pack(scenes.icosphere_tris(seed=12, level=4))
+    inst = scenes.pack(scenes.instanced(n_instances=48, tex_size=64))
+    return [(sf, abi.MODE_PATH_TRACE), (ico, abi.MODE_RAY_TRACE), (inst, abi.MODE_RAY_TRACE)]
+
+
+def plain(packed, cfg, mode):
+    with Renderer().Initialize() as r:
+        r.LoadScene(packed)
+        r.RenderFrame(cfg, mode)
+        return r.ReadImage(), r.ReadDisplay()
+
+
+def test_job_alone_equals_plain_render(scene_set):
+    say("served job alone vs avrt_render")
+    pk, mode = scene_set[0]
+    cfg = serve.job_config(pk, 160, 90, 5, 6)
+    with serve.Server() as srv:
+        sid = srv.add_scene(pk)
+        jid = srv.submit(sid, cfg, mode)
+        while srv.status(jid).state != abi.JOB_DONE:
+            srv.step()
+        img, disp = srv.read(jid)
+    ref, rdisp = plain(pk, cfg, mode)
+    assert np.array_equal(img, ref)
+
+
+def test_jobs_in_a_mix_equal_plain_renders(scene_set):
+    """Uneven jobs batched together; a small round capacity splits the
+    larger jobs' samples over several rounds."""
+    say("served mix vs avrt_render")
+    specs = [(0, 160, 90, 3, 5), (1, 200, 120, 2, 4), (2, 96, 64, 4, 8), (1, 64, 64, 7, 6), (0, 128, 72, 1, 4),
+             (2, 180, 100, 3, 5)]
+    with serve.Server(round_paths=60000) as srv:
+        sids = [srv.add_scene(pk) for pk, _ in scene_set]
+        jobs = []
+        for sc, w, h, spp, depth in specs:
+            pk, mode = scene_set[sc]
+            jobs.append(srv.submit(sids[sc], serve.job_config(pk, w, h, spp, depth), mode))
+        rounds = 0
+        while any(srv.status(j).state != abi.JOB_DONE for j in jobs):
+            srv.step()
+            rounds += 1
+            assert rounds < 200
+        st = srv.stats()
+        say(f"rounds {rounds}, stats {st}")
+        assert st["rounds"] >= 3
+        out = [srv.read(j) for j in jobs]
+    for (sc, w, h, spp, depth), (img, disp) in zip(specs, out):
+        pk, mode = scene_set[sc]
+        ref, rdisp = plain(pk, serve.job_config(pk, w, h, spp, depth), mode)
+        assert np.array_equal(img, ref), (sc, w, h, spp, depth)
+        if mode == abi.MODE_RAY_TRACE:
+            assert np.array_equal(disp, rdisp)
+
+
+def test_admission_waits_for_budget(scene_set):
+    """Budget = round queues + scene + one 256x256 job's targets: the second
+    job waits in the queue until the first is finished and released."""
+    say("admission by HBM budget")
+    pk, mode = scene_set[1]
+    with pytest.raises(abi.AvrtError):
+        serve.Server(hbm_budget_gib=1e-4, round_paths=1 << 18)  # the round queues alone exceed it
+    with serve.Server(round_paths=1 << 18) as probe:
+        probe.add_scene(pk)
+        base = probe.stats()["hbm_used_bytes"]
+    job_bytes = 256 * 256 * 20
+    with serve.Server(hbm_budget_gib=(base + 1.5 * job_bytes) / (1 << 30), round_paths=1 << 18) as srv:
+        sid = srv.add_scene(pk)
+        with pytest.raises(abi.AvrtError):
+            srv.submit(sid, serve.job_config(pk, 1920, 1080, 1, 4), mode)  # one frame > round capacity
+        a = srv.submit(sid, serve.job_config(pk, 256, 256, 2, 4), mode)
+        b = srv.submit(sid, serve.job_config(pk, 256, 256, 1, 4), mode)
+        srv.step()
+        st = srv.stats()
+        assert st["queued"] == 1 and srv.status(b).state == abi.JOB_QUEUED
+        while srv.status(a).state != abi.JOB_DONE:
+            srv.step()
+        srv.step()
+        assert srv.status(b).state == abi.JOB_QUEUED  # a's targets still held
+        srv.release(a)
+        while srv.status(b).state != abi.JOB_DONE:
+            srv.step()
+        assert srv.stats()["completed"] == 2
+
+
+def test_release_of_queued_job(scene_set):
+    pk, mode = scene_set[0]
+    with serve.Server(max_jobs=1) as srv:
+        sid = srv.add_scene(pk)
+        a = srv.submit(sid, serve.job_config(pk, 64, 64, 1, 4), mode)
+        b = srv.submit(sid, serve.job_config(pk, 64, 64, 1, 4), mode)
+        srv.release(b)
+        while srv.status(a).state != abi.JOB_DONE:
+            srv.step()
+        assert srv.stats()["completed"] == 1
