@@ -63,6 +63,10 @@ def main():
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lane-trace", action="store_true", help="one ray per lane instead of persistent lanes")
+    ap.add_argument("--served-jobs", type=int, default=160, help="jobs of the served stream (0 skips the leg)")
+    ap.add_argument("--served-load", type=float, default=0.6,
+                    help="Poisson arrival rate of the latency run, as a fraction of the measured served capacity")
+    ap.add_argument("--hybrid-frames", type=int, default=16, help="frames of the hybrid orbit (0 skips the leg)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -134,6 +138,9 @@ def main():
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_extend",
                     "bytes_per_ray": b_ext, "avg_launch_ms": per_launch_ms}
 
+    served = run_served(args, mrays) if args.served_jobs > 0 else None
+    hyb = run_hybrid(args) if args.hybrid_frames > 0 else None
+
     out = {
         "metric": "Mrays/s, 1080p two-level BVH, served vs plain; p50/p99 job latency at fixed load",
         "value": mrays,
@@ -156,8 +163,89 @@ def main():
                                                                 "ms_other")},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "served": served,
+        "hybrid": hyb,
     }
     print(json.dumps(out))
+
+
+def run_served(args, plain_mrays):
+    """The served, batched form on the mixed stream: 8 seeded scenes shaped
+    like the sphere field (path_trace) and the icosphere triangle scene
+    (ray_trace), resolution uniform over {256^2, 640x360, 1280x720,
+    1920x1080}, samples log-uniform in [1, 64], depth 4..8. Throughput: every
+    job queued at once, rays / wall time until the last finishes. Latency: a
+    Poisson stream at `served_load` x the measured capacity (jobs/s), p50/p99
+    of arrival -> result."""
+    import numpy as np
+
+    from aviatorengine2_amd import serve
+
+    t0 = time.perf_counter()
+    scene_list = serve.mixed_scenes(8)
+    log(f"served: 8 scenes built in {time.perf_counter() - t0:.1f}s")
+    jobs = serve.poisson_jobs(args.served_jobs, rate_hz=1e9, seed=0x10B5)
+    with serve.Server() as srv:
+        ids = [srv.add_scene(pk) for pk, _ in scene_list]
+        # warm the kernels once
+        warm = serve.poisson_jobs(4, rate_hz=1e9, seed=7, max_spp=2)
+        serve.run_stream(srv, ids, scene_list, warm)
+        s0 = srv.stats()
+        lat, wall, st, _ = serve.run_stream(srv, ids, scene_list, jobs, progress=log)
+        rays = (st["rays_primary"] + st["rays_bounce"] + st["rays_shadow"]) - (
+            s0["rays_primary"] + s0["rays_bounce"] + s0["rays_shadow"])
+        cap_jobs = len(jobs) / wall
+        mrays = rays / wall / 1e6
+        samples = (st["paths"] - s0["paths"]) / wall
+        log(f"served throughput: {mrays:.1f} Mrays/s, {cap_jobs:.1f} jobs/s over {wall:.1f}s, {st['rounds'] - s0['rounds']} rounds")
+        rate = args.served_load * cap_jobs
+        ljobs = serve.poisson_jobs(args.served_jobs, rate_hz=rate, seed=0x10B6)
+        llat, lwall, lst, _ = serve.run_stream(srv, ids, scene_list, ljobs, progress=log)
+        p50, p99 = float(np.percentile(llat, 50)) * 1e3, float(np.percentile(llat, 99)) * 1e3
+        log(f"served latency at {rate:.1f} jobs/s: p50 {p50:.1f} ms, p99 {p99:.1f} ms")
+    return {"mrays_per_s": mrays, "vs_plain": mrays / plain_mrays if plain_mrays else None, "samples_per_s": samples,
+            "jobs": len(jobs), "jobs_per_s": cap_jobs, "wall_s": wall,
+            "latency": {"rate_jobs_per_s": rate, "load": args.served_load, "p50_ms": p50, "p99_ms": p99,
+                        "jobs": len(ljobs), "wall_s": lwall},
+            "stream": "8 scenes (4 sphere fields path_trace, 4 icosphere triangle scenes ray_trace), "
+                      "res {256^2,640x360,1280x720,1920x1080}, spp log-uniform [1,64], depth 4-8"}
+
+
+def run_hybrid(args):
+    """The hybrid frame on the icosphere scene in two-level form: ray-cast
+    G-buffer at 1920x1080, deferred lighting on the 960x540 trace grid (1
+    sample: NEE + bounce path, depth 5), upsample + reprojection to 1080p,
+    over an orbit of `hybrid_frames` frames at 1 degree per frame around the
+    scene at the scene camera's own distance."""
+    import math
+
+    from aviatorengine2_amd import hybrid, scenes
+    from aviatorengine2_amd.renderer import Renderer
+
+    spec = scenes.hybrid_scene(width=1920, height=1080, spp=1, depth=5, tex_size=1024)
+    pk = scenes.pack(spec)
+    c = pk.config
+    src = (float(c["source_x"]), float(c["source_y"]), float(c["source_z"]))
+    radius = math.hypot(src[0], src[2])
+    with Renderer().Initialize() as r:
+        r.LoadScene(pk)
+        frames = hybrid.HybridFrames(r, c, trace_scale=0.5)
+        frames.render()  # warm-up frame
+        rays = 0
+        dev_ms = 0.0
+        t0 = time.perf_counter()
+        for f in range(args.hybrid_frames):
+            frames.set_camera(hybrid.orbit((0.0, 0.0, 0.0), radius, src[1], 1.0 * (f + 1)), (0.0, 0.0, 0.0))
+            _, st = frames.render()
+            rays += st["rays_primary"] + st["rays_bounce"] + st["rays_shadow"]
+            dev_ms += st["ms_total"]
+        wall = time.perf_counter() - t0
+    g = args.hybrid_frames
+    log(f"hybrid: {1e3 * wall / g:.2f} ms/frame wall, {dev_ms / g:.2f} ms device, {rays / wall / 1e6:.1f} Mrays/s")
+    return {"ms_per_frame": 1e3 * wall / g, "device_ms_per_frame": dev_ms / g, "mrays_per_s": rays / wall / 1e6,
+            "frames": g, "gbuffer": "1920x1080", "trace_grid": "960x540", "output": "1920x1080",
+            "workload": "icosphere scene (81,920-tri mesh + ground mesh, quad light, glass/metal spheres, proxy box), "
+                        "orbit 1 deg/frame"}
 
 
 if __name__ == "__main__":
