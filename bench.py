@@ -48,6 +48,19 @@ def bytes_per_ray(counts):
     return (48 * nodes + 128 * objs + 48 * insts + 108 * tris) / rays + 64.0
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the extension traversal, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (scaled
+    by the copy-kernel calibration factors), when this run is that workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    default = (args.width, args.height, args.spp, args.depth, args.instances) == (1920, 1080, 8, 8, 1024)
+    if not default or args.lane_trace or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return 1024.0 * (t["fetch_kib_per_launch"] * t["fetch_factor"] + t["write_kib_per_launch"] * t["write_factor"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +148,8 @@ def main():
         per_launch_ms = tot["ms_extend"] / launches
         achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_extend",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                    "kernel": "k_trace<false, false>" if not args.lane_trace else "k_trace_lane<false, false>",
                     "bytes_per_ray": b_ext, "avg_launch_ms": per_launch_ms}
 
     served = run_served(args, mrays) if args.served_jobs > 0 else None

@@ -379,7 +379,6 @@ int avrt_update_vertices(avrt_ctx *ctx, const AvrtVertex *vertices,
                          uint32_t first, uint32_t count);
 int avrt_read_blas(avrt_ctx *ctx, AvrtBVHNode *dst, uint32_t count);
 
-/* Library version string. */
 /* ------------------------------------------------------------------------- */
 /* Serving: render jobs of resident scenes, batched across jobs.             */
 /* ------------------------------------------------------------------------- */
@@ -442,6 +441,7 @@ int avrt_server_read_job(avrt_server *srv, uint64_t job_id, float *image, size_t
 int avrt_server_release_job(avrt_server *srv, uint64_t job_id);
 int avrt_server_get_stats(const avrt_server *srv, AvrtServerStats *out);
 
+/* Library version string. */
 const char *avrt_version(void);
 
 #ifdef __cplusplus
