@@ -83,11 +83,20 @@ struct avrt_ctx {
     std::vector<uint32_t> h_indices;
     std::vector<AvrtBVHNode> h_blas;
     std::vector<avrt::Node4> h_blas2;
+    std::vector<uint32_t> h_blas2_src;               // per Node4 child slot: source node in h_blas
+    // BLAS refit schedule: nodes of every prepared BLAS, deepest level first,
+    // with the mesh (vertexBase, indexBase) each belongs to.
+    std::vector<uint32_t> refit_node;                // global node index into h_blas
+    std::vector<uint32_t> refit_depth;
+    std::vector<uint32_t> refit_tree;                // uint32[3] per node: tree base, vertexBase, indexBase
+    std::vector<uint32_t> refit_meshes;              // uint32[4] per mesh: vertexBase, indexBase, triangles, tree base
+    bool refit_dirty = true;                         // schedule changed since the device copy
     std::vector<avrt::TriPre> h_tris;
     std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
     std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase
     bool geometry_loaded = false;
     avrt::DevBuf d_verts, d_indices, d_blas2, d_tris;
+    avrt::DevBuf d_blas, d_blas2_src, d_refit_node, d_refit_tree, d_refit_box;
 
     // scene as the renderer reads it (slot order)
     uint32_t nobj = 0, nlights = 0, nquad_lights = 0, ntlas2 = 0;

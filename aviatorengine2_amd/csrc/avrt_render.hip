@@ -232,9 +232,26 @@ int avrt::prepare_meshes(avrt_ctx *c, const AvrtMeshInstance *inst, uint32_t nin
             uint32_t super = 0;
             std::string why;
             if (!convert_tree(c->h_blas.data(), m.blasBase, static_cast<uint32_t>(c->h_blas.size() - m.blasBase), c->h_blas2,
-                              &super, &why))
+                              &super, &why, nullptr, 0, &c->h_blas2_src))
                 return set_err(c, AVRT_E_INVALID_ARG, "BLAS at " + std::to_string(m.blasBase) + ": " + why);
             c->blas_root[m.blasBase] = super;
+            // refit schedule: this tree's nodes with their depths
+            std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
+            while (!todo.empty()) {
+                const auto [local, d] = todo.back();
+                todo.pop_back();
+                const uint32_t g = m.blasBase + local;
+                c->refit_node.push_back(g);
+                c->refit_depth.push_back(d);
+                c->refit_tree.insert(c->refit_tree.end(), {m.blasBase, m.vertexBase, m.indexBase});
+                const AvrtBVHNode &n = c->h_blas[g];
+                if (n.count == 0) {
+                    todo.push_back({static_cast<uint32_t>(n.left), d + 1});
+                    todo.push_back({static_cast<uint32_t>(n.right), d + 1});
+                }
+            }
+            c->refit_meshes.insert(c->refit_meshes.end(), {m.vertexBase, m.indexBase, m.triangleCount, m.blasBase});
+            c->refit_dirty = true;
             *changed = true;
         }
         auto it = c->tri_vbase.find(m.indexBase);
@@ -346,7 +363,8 @@ int avrt_destroy(avrt_ctx *c)
     if (c == nullptr) return AVRT_E_INVALID_ARG;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects, &c->d_geom, &c->d_tlas2,
+    DevBuf *bufs[] = {&c->d_blas, &c->d_blas2_src, &c->d_refit_node, &c->d_refit_tree, &c->d_refit_box,
+                      &c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects, &c->d_geom, &c->d_tlas2,
                       &c->d_lights, &c->d_instances, &c->d_tex, &c->d_image, &c->d_display, &c->so[0], &c->so[1],
                       &c->sd[0], &c->sd[1], &c->st[0], &c->st[1], &c->hitA, &c->hitB, &c->Lpath, &c->shp, &c->shn,
                       &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
@@ -380,6 +398,12 @@ int avrt_upload_geometry(avrt_ctx *c, const AvrtVertex *vertices, uint32_t verte
     c->h_indices.assign(indices, indices + index_count);
     c->h_blas.assign(blas, blas + blas_count);
     c->h_blas2.clear();
+    c->h_blas2_src.clear();
+    c->refit_node.clear();
+    c->refit_depth.clear();
+    c->refit_tree.clear();
+    c->refit_meshes.clear();
+    c->refit_dirty = true;
     c->h_tris.clear();
     c->blas_root.clear();
     c->tri_vbase.clear();

@@ -890,6 +890,169 @@ int run_rebuild(avrt_ctx *c, const float *vel_host, float dt)
     return AVRT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// BLAS refit: node boxes from the current vertices, the builder's rules
+// (BuildRecursive box = Surround of the primitives' AABBs, triangle AABB =
+// Surround2Points + SurroundPoint, MakeNode inflation; scene/bvh.h:58-89,
+// scene/mesh_library.h:120-135), deepest level first; then the Node4 boxes
+// and the triangle records.
+// ---------------------------------------------------------------------------
+struct Box6 {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ void box_grow(Box6 &b, float x, float y, float z)
+{
+    b.lo[0] = fminf(b.lo[0], x);
+    b.lo[1] = fminf(b.lo[1], y);
+    b.lo[2] = fminf(b.lo[2], z);
+    b.hi[0] = fmaxf(b.hi[0], x);
+    b.hi[1] = fmaxf(b.hi[1], y);
+    b.hi[2] = fmaxf(b.hi[2], z);
+}
+
+__global__ void k_refit_level(uint32_t n, const uint32_t *__restrict__ node, const uint32_t *__restrict__ tree,
+                              const AvrtVertex *__restrict__ verts, const uint32_t *__restrict__ indices,
+                              AvrtBVHNode *blas, Box6 *box)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t g = node[k];
+    const uint32_t base = tree[3 * k], vbase = tree[3 * k + 1], ibase = tree[3 * k + 2];
+    AvrtBVHNode nd = blas[g];
+    Box6 b;
+    for (int a = 0; a < 3; a++) {
+        b.lo[a] = 3.402823466e38f;
+        b.hi[a] = -3.402823466e38f;
+    }
+    if (nd.count > 0) {
+        for (int i = 0; i < nd.count; i++) {
+            const uint32_t t3 = ibase + static_cast<uint32_t>(nd.left + i) * 3u;
+            for (int v = 0; v < 3; v++) {
+                const AvrtVertex p = verts[vbase + indices[t3 + v]];
+                box_grow(b, p.px, p.py, p.pz);
+            }
+        }
+    } else {
+        const Box6 l = box[base + static_cast<uint32_t>(nd.left)], r = box[base + static_cast<uint32_t>(nd.right)];
+        box_grow(b, l.lo[0], l.lo[1], l.lo[2]);
+        box_grow(b, l.hi[0], l.hi[1], l.hi[2]);
+        box_grow(b, r.lo[0], r.lo[1], r.lo[2]);
+        box_grow(b, r.hi[0], r.hi[1], r.hi[2]);
+    }
+    box[g] = b;
+    const float bm = 0.0001f;
+    float lo[3] = {b.lo[0], b.lo[1], b.lo[2]}, hi[3] = {b.hi[0], b.hi[1], b.hi[2]};
+    for (int a = 0; a < 3; a++) {
+        if (hi[a] - lo[a] < bm) {
+            lo[a] -= bm * 0.5f;
+            hi[a] += bm * 0.5f;
+        }
+    }
+    nd.min_x = lo[0];
+    nd.min_y = lo[1];
+    nd.min_z = lo[2];
+    nd.max_x = hi[0];
+    nd.max_y = hi[1];
+    nd.max_z = hi[2];
+    blas[g] = nd;
+}
+
+__global__ void k_refit_node4(uint32_t n4, const uint32_t *__restrict__ src, const AvrtBVHNode *__restrict__ blas,
+                              Node4 *out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 4 * n4) return;
+    const uint32_t g = src[k];
+    if (g == ~0u) return;
+    const AvrtBVHNode b = blas[g];
+    Node4 &nd = out[k >> 2];
+    const int ch = static_cast<int>(k & 3u);
+    nd.lox[ch] = b.min_x;
+    nd.hix[ch] = b.max_x;
+    nd.loy[ch] = b.min_y;
+    nd.hiy[ch] = b.max_y;
+    nd.loz[ch] = b.min_z;
+    nd.hiz[ch] = b.max_z;
+}
+
+// make_tripre (scene_convert.cpp) on the device: the same float operations.
+__global__ void k_refit_tris(uint32_t vbase, uint32_t ibase, uint32_t count, const AvrtVertex *__restrict__ verts,
+                             const uint32_t *__restrict__ indices, TriPre *tris)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const uint32_t *ix = indices + ibase + 3u * s;
+    const AvrtVertex a = verts[vbase + ix[0]], b = verts[vbase + ix[1]], c = verts[vbase + ix[2]];
+    const dev::F3 p0 = dev::f3(a.px, a.py, a.pz);
+    const dev::F3 e1 = dev::f3(b.px - a.px, b.py - a.py, b.pz - a.pz);
+    const dev::F3 e2 = dev::f3(c.px - a.px, c.py - a.py, c.pz - a.pz);
+    const dev::F3 n = dev::cross(e1, e2);
+    const float nl = dev::dot(n, n);
+    TriPre t;
+    t.p0[0] = p0.x; t.p0[1] = p0.y; t.p0[2] = p0.z;
+    t.e1[0] = e1.x; t.e1[1] = e1.y; t.e1[2] = e1.z;
+    t.e2[0] = e2.x; t.e2[1] = e2.y; t.e2[2] = e2.z;
+    t.w[0] = n.x / nl; t.w[1] = n.y / nl; t.w[2] = n.z / nl;
+    t.d0 = dev::dot(n, p0);
+    t.valid = (nl < 1e-12f) ? 0u : 1u;
+    t.pad0 = 0.0f;
+    t.pad1 = 0.0f;
+    tris[ibase / 3u + s] = t;
+}
+
+int refit(avrt_ctx *c)
+{
+    const size_t nn = c->refit_node.size();
+    int rc = 0;
+    if (c->refit_dirty) {
+        c->h_blas2_src.resize(4 * c->h_blas2.size(), ~0u);
+        // deepest level first
+        std::vector<uint32_t> order(nn);
+        for (size_t i = 0; i < nn; i++) order[i] = static_cast<uint32_t>(i);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return c->refit_depth[a] > c->refit_depth[b]; });
+        std::vector<uint32_t> node(nn), tree(3 * nn);
+        for (size_t i = 0; i < nn; i++) {
+            node[i] = c->refit_node[order[i]];
+            for (int k = 0; k < 3; k++) tree[3 * i + k] = c->refit_tree[3 * order[i] + k];
+        }
+        if ((rc = upload_vec(c, c->d_refit_node, node.data(), 4 * nn)) ||
+            (rc = upload_vec(c, c->d_refit_tree, tree.data(), 12 * nn)) ||
+            (rc = upload_vec(c, c->d_blas, c->h_blas.data(), sizeof(AvrtBVHNode) * c->h_blas.size())) ||
+            (rc = upload_vec(c, c->d_blas2_src, c->h_blas2_src.data(), 4 * c->h_blas2_src.size())) ||
+            (rc = ensure(c, c->d_refit_box, sizeof(Box6) * c->h_blas.size())))
+            return rc;
+        c->refit_dirty = false;
+    }
+    std::vector<uint32_t> depth_sorted(c->refit_depth);
+    std::sort(depth_sorted.begin(), depth_sorted.end(), std::greater<uint32_t>());
+    size_t i = 0;
+    while (i < nn) {
+        size_t j = i;
+        while (j < nn && depth_sorted[j] == depth_sorted[i]) j++;
+        const uint32_t cnt = static_cast<uint32_t>(j - i);
+        hipLaunchKernelGGL(k_refit_level, dim3(blocks(cnt, 256)), dim3(256), 0, c->stream, cnt,
+                           c->d_refit_node.as<uint32_t>() + i, c->d_refit_tree.as<uint32_t>() + 3 * i,
+                           c->d_verts.as<AvrtVertex>(), c->d_indices.as<uint32_t>(), c->d_blas.as<AvrtBVHNode>(),
+                           c->d_refit_box.as<Box6>());
+        HIPX(c, hipGetLastError());
+        i = j;
+    }
+    const uint32_t n4 = static_cast<uint32_t>(c->h_blas2.size());
+    hipLaunchKernelGGL(k_refit_node4, dim3(blocks(4 * n4, 256)), dim3(256), 0, c->stream, n4,
+                       c->d_blas2_src.as<uint32_t>(), c->d_blas.as<AvrtBVHNode>(), c->d_blas2.as<Node4>());
+    HIPX(c, hipGetLastError());
+    for (size_t m = 0; m + 3 < c->refit_meshes.size(); m += 4) {
+        const uint32_t vb = c->refit_meshes[m], ib = c->refit_meshes[m + 1], tc = c->refit_meshes[m + 2];
+        hipLaunchKernelGGL(k_refit_tris, dim3(blocks(tc, 256)), dim3(256), 0, c->stream, vb, ib, tc,
+                           c->d_verts.as<AvrtVertex>(), c->d_indices.as<uint32_t>(), c->d_tris.as<TriPre>());
+        HIPX(c, hipGetLastError());
+    }
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -998,11 +1161,41 @@ int avrt_scene_read(avrt_ctx *c, AvrtBVHNode *nodes, uint32_t *node_count, AvrtO
     return AVRT_OK;
 }
 
-int avrt_refit_blas(avrt_ctx *c) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
-int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *, uint32_t, uint32_t)
+int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *vertices, uint32_t first, uint32_t count)
 {
-    return set_err(c, AVRT_E_UNSUPPORTED, "not implemented");
+    if (c == nullptr || (count && vertices == nullptr)) return AVRT_E_INVALID_ARG;
+    if (!c->geometry_loaded) return set_err(c, AVRT_E_STATE, "vertices before avrt_upload_geometry");
+    if (static_cast<uint64_t>(first) + count > c->h_verts.size())
+        return set_err(c, AVRT_E_INVALID_ARG, "vertex range beyond the uploaded vertex buffer");
+    HIPX(c, hipSetDevice(c->device));
+    std::copy(vertices, vertices + count, c->h_verts.begin() + first);
+    if (count)
+        HIPX(c, hipMemcpyAsync(c->d_verts.as<AvrtVertex>() + first, vertices, sizeof(AvrtVertex) * count,
+                               hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
 }
-int avrt_read_blas(avrt_ctx *c, AvrtBVHNode *, uint32_t) { return set_err(c, AVRT_E_UNSUPPORTED, "not implemented"); }
+
+int avrt_refit_blas(avrt_ctx *c)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    if (c->blas_root.empty()) return set_err(c, AVRT_E_STATE, "no BLAS prepared (upload a scene with mesh instances)");
+    HIPX(c, hipSetDevice(c->device));
+    return refit(c);
+}
+
+int avrt_read_blas(avrt_ctx *c, AvrtBVHNode *dst, uint32_t count)
+{
+    if (c == nullptr || (count && dst == nullptr)) return AVRT_E_INVALID_ARG;
+    if (count > c->h_blas.size()) return set_err(c, AVRT_E_CAPACITY, "more nodes than the uploaded BLAS buffer holds");
+    HIPX(c, hipSetDevice(c->device));
+    if (c->d_blas.p == nullptr) {
+        std::copy(c->h_blas.begin(), c->h_blas.begin() + count, dst);
+        return AVRT_OK;
+    }
+    HIPX(c, hipMemcpyAsync(dst, c->d_blas.p, sizeof(AvrtBVHNode) * count, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
 
 } // extern "C"

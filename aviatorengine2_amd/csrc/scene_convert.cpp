@@ -29,7 +29,15 @@ struct Converter {
     std::vector<Node4> *out;
     const uint8_t *slot_class; // null: every subtree holds both classes
     uint32_t nslots;
+    std::vector<uint32_t> *src; // optional: source node (base-relative + base) of each child box
     std::string err;
+
+    void note_src(uint32_t me, int c, uint32_t node)
+    {
+        if (src == nullptr) return;
+        if (src->size() < 4 * static_cast<size_t>(me + 1)) src->resize(4 * static_cast<size_t>(me + 1), ~0u);
+        (*src)[4 * static_cast<size_t>(me) + c] = node;
+    }
 
     static void set_child(Node4 &n, int c, const float lo[3], const float hi[3], uint32_t ref)
     {
@@ -73,7 +81,8 @@ struct Converter {
 
     // Reference for a leaf [first, first+count) whose box is lo/hi. Leaves
     // over 127 primitives become small subtrees over the same box.
-    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref, uint32_t *cls)
+    bool leaf_ref(uint32_t first, uint32_t count, const float lo[3], const float hi[3], uint32_t *ref, uint32_t *cls,
+                  uint32_t node)
     {
         if (count == 0) {
             err = "empty leaf";
@@ -92,10 +101,12 @@ struct Converter {
         out->push_back(empty_node());
         const uint32_t half = count / 2;
         uint32_t r0, r1, c0, c1;
-        if (!leaf_ref(first, half, lo, hi, &r0, &c0)) return false;
-        if (!leaf_ref(first + half, count - half, lo, hi, &r1, &c1)) return false;
+        if (!leaf_ref(first, half, lo, hi, &r0, &c0, node)) return false;
+        if (!leaf_ref(first + half, count - half, lo, hi, &r1, &c1, node)) return false;
         set_child((*out)[me], 0, lo, hi, r0);
         set_child((*out)[me], 1, lo, hi, r1);
+        note_src(me, 0, node);
+        note_src(me, 1, node);
         (*out)[me].pad[0] = child_mask_bits(0, c0) | child_mask_bits(1, c1);
         *ref = me;
         *cls = c0 | c1;
@@ -118,7 +129,7 @@ struct Converter {
                 err = "negative leaf slot";
                 return false;
             }
-            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref, cls);
+            return leaf_ref(static_cast<uint32_t>(n.left), static_cast<uint32_t>(n.count), lo, hi, ref, cls, base + local);
         }
         if (n.left < 0 || n.right < 0 || static_cast<uint32_t>(n.left) >= avail || static_cast<uint32_t>(n.right) >= avail) {
             err = "bad child index";
@@ -148,6 +159,7 @@ struct Converter {
             float lo[3], hi[3];
             if (!box_of(kids[c], lo, hi)) return false;
             set_child((*out)[me], c, lo, hi, r);
+            note_src(me, c, base + kids[c]);
             mask |= child_mask_bits(c, cc);
             all |= cc;
         }
@@ -168,6 +180,7 @@ struct Converter {
         float lo[3], hi[3];
         box_of(0, lo, hi);
         set_child((*out)[me], 0, lo, hi, r);
+        note_src(me, 0, base);
         (*out)[me].pad[0] = child_mask_bits(0, cls);
         *super = me;
         return true;
@@ -191,9 +204,10 @@ uint8_t object_class(const AvrtObject &o)
 }
 
 bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
-                  std::string *err, const uint8_t *slot_class, uint32_t nslots)
+                  std::string *err, const uint8_t *slot_class, uint32_t nslots, std::vector<uint32_t> *src)
 {
-    Converter cv{nodes, base, avail, &out, slot_class, nslots, {}};
+    if (src) src->resize(4 * out.size(), ~0u);
+    Converter cv{nodes, base, avail, &out, slot_class, nslots, src, {}};
     if (!cv.convert(super)) {
         *err = cv.err;
         return false;

@@ -17,9 +17,12 @@ namespace avrt {
 // `out`; *super receives the super-root's index. Leaves of more than 127
 // primitives become small subtrees over the same box. `slot_class` (kClassLG
 // / kClassOther per leaf slot, `nslots` of them) fills the child masks; null
-// marks every child as holding both classes.
+// marks every child as holding both classes. `src`, when given, receives for
+// every child slot of every Node4 in `out` the index in `nodes` of the node
+// whose box fills it (~0u for empty slots), which is what a refit needs.
 bool convert_tree(const AvrtBVHNode *nodes, uint32_t base, uint32_t avail, std::vector<Node4> &out, uint32_t *super,
-                  std::string *err, const uint8_t *slot_class = nullptr, uint32_t nslots = 0);
+                  std::string *err, const uint8_t *slot_class = nullptr, uint32_t nslots = 0,
+                  std::vector<uint32_t> *src = nullptr);
 
 // IsRasterProxy or a mesh: what the hybrid renderer rasterizes
 // (deferred.comp.hlsl:1011-1017, renderer.h IsRasterProxy).

@@ -158,6 +158,19 @@ class Renderer:
                     "avrt_trace_primary")
         return obj, prim, t
 
+    # -- deformable meshes: vertex edits and BLAS refit ----------------------
+    def UpdateVertices(self, vertices, first=0):
+        v = np.ascontiguousarray(vertices, abi.VERTEX_DTYPE)
+        self._check(self.lib.avrt_update_vertices(self.ctx, ptr(v), int(first), len(v)), "avrt_update_vertices")
+
+    def RefitBLAS(self):
+        self._check(self.lib.avrt_refit_blas(self.ctx), "avrt_refit_blas")
+
+    def ReadBLAS(self, count):
+        out = np.zeros(count, abi.NODE_DTYPE)
+        self._check(self.lib.avrt_read_blas(self.ctx, ptr(out), count), "avrt_read_blas")
+        return out
+
     # -- device-side scene rebuild (RebuildAcceleration on the GPU) ----------
     def SetAuthoring(self, objects, instances, local_bounds):
         """Objects in authoring (GatherRenderables) order, one instance record

@@ -371,12 +371,20 @@ int avrt_scene_read(avrt_ctx *ctx, AvrtBVHNode *nodes, uint32_t *node_count,
                     AvrtObject *objects, AvrtMeshInstance *instances,
                     uint32_t *order);
 
-/* BLAS refit on the device after vertex positions changed: recomputes every
- * BLAS node box bottom-up from the current vertices (same inflation rule as
- * the builder, scene/bvh.h:58-89). Topology is unchanged. */
-int avrt_refit_blas(avrt_ctx *ctx);
+/* Deformable meshes (no counterpart in the reference, whose BLASes are built
+ * once in MeshLibrary::Add, scene/mesh_library.h:91-159). avrt_update_vertices
+ * replaces vertices [first, first+count) of the uploaded vertex buffer;
+ * avrt_refit_blas then recomputes, on the device, every node box of every
+ * BLAS the scene's instances use from the current vertices with the
+ * builder's rules (box = Surround of the triangles' AABBs, MakeNode
+ * inflation; scene/bvh.h:58-89), deepest level first, plus the traversal
+ * copies of the boxes and triangles. Topology is unchanged. The TLAS keeps
+ * the world bounds it was built from: re-upload or re-step the scene when
+ * meshes grow. avrt_read_blas copies the first `count` BLAS nodes (the
+ * reference layout) back. */
 int avrt_update_vertices(avrt_ctx *ctx, const AvrtVertex *vertices,
                          uint32_t first, uint32_t count);
+int avrt_refit_blas(avrt_ctx *ctx);
 int avrt_read_blas(avrt_ctx *ctx, AvrtBVHNode *dst, uint32_t count);
 
 /* ------------------------------------------------------------------------- */

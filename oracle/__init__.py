@@ -33,6 +33,7 @@ _SIG = {
     "orc_closest_fp64": (None, [_P, _U32, _P, _U32, _P, _P, _P, _P, _P, _P]),
     "orc_sizeof_config": (ctypes.c_int, []),
     "orc_halton": (ctypes.c_float, [_U32, _U32]),
+    "orc_refit_blas": (None, [_P, _P, _P, _U32, _U32]),
     "orc_exp": (ctypes.c_float, [ctypes.c_float]),
     "orc_gbuffer": (None, [_P, _P, _U32, _P, _U32, _P, _P, _P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P]),
     "orc_hybrid_frame": (None, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _P, _P, _P, _P, _U32, _U32, _U32, _P, _P, _P,
@@ -177,3 +178,12 @@ def hybrid_frame(packed, config, gbuf, prev_img, prev_geom, fp16=False, threads=
                          _p(prev_img), _p(prev_geom), _p(img), _p(geom), _p(disp), _p(lred), 1 if fp16 else 0,
                          int(threads))
     return img, geom, disp, lred
+
+
+def refit_blas(nodes, vertices, indices, vertex_base, index_base):
+    """A refit copy of one BLAS (nodes relative to its root)."""
+    out = np.array(nodes, copy=True)
+    v = np.ascontiguousarray(vertices)
+    ix = np.ascontiguousarray(indices, np.uint32)
+    load().orc_refit_blas(_p(out), _p(v), _p(ix), int(vertex_base), int(index_base))
+    return out

@@ -1647,6 +1647,40 @@ void orc_instance_bounds(const Inst *inst, uint32_t n, const AABB *local, AABB *
     }
 }
 
+// Refit of one BLAS (nodes relative to their root, as MeshLibrary::Add stores
+// them) from the current vertices: each node's box is the Surround of its
+// triangles' AABBs (Surround2Points + SurroundPoint), inflated by MakeNode's
+// rule (scene/bvh.h:58-89, scene/mesh_library.h:120-135); children first.
+static AABB refit_rec(Node *nodes, int i, const Vertex *verts, const uint32_t *idx, uint32_t vb, uint32_t ib)
+{
+    Node &n = nodes[i];
+    AABB b = empty_box();
+    if (n.count > 0) {
+        for (int k = 0; k < n.count; k++) {
+            const uint32_t t3 = ib + static_cast<uint32_t>(n.left + k) * 3u;
+            const Vertex &a = verts[vb + idx[t3]], &bb = verts[vb + idx[t3 + 1]], &c = verts[vb + idx[t3 + 2]];
+            AABB e{std::min(a.px, bb.px), std::min(a.py, bb.py), std::min(a.pz, bb.pz),
+                   std::max(a.px, bb.px), std::max(a.py, bb.py), std::max(a.pz, bb.pz)};
+            e = AABB{std::min(e.min_x, c.px), std::min(e.min_y, c.py), std::min(e.min_z, c.pz),
+                     std::max(e.max_x, c.px), std::max(e.max_y, c.py), std::max(e.max_z, c.pz)};
+            b = surround(b, e);
+        }
+    } else {
+        const AABB l = refit_rec(nodes, n.left, verts, idx, vb, ib);
+        const AABB r = refit_rec(nodes, n.right, verts, idx, vb, ib);
+        b = surround(l, r);
+    }
+    const Node m = make_node(b, n.left, n.right, n.count);
+    n.min_x = m.min_x; n.min_y = m.min_y; n.min_z = m.min_z;
+    n.max_x = m.max_x; n.max_y = m.max_y; n.max_z = m.max_z;
+    return b;
+}
+
+void orc_refit_blas(Node *nodes, const Vertex *verts, const uint32_t *idx, uint32_t vertex_base, uint32_t index_base)
+{
+    refit_rec(nodes, 0, verts, idx, vertex_base, index_base);
+}
+
 void orc_sincos(float a, float *s, float *c) { rt_sincos(a, s, c); }
 
 float orc_random(uint32_t *seed)

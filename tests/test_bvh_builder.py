@@ -15,6 +15,7 @@ Pins:
 import numpy as np
 import pytest
 
+import oracle
 from aviatorengine2_amd import abi, scenes
 
 F32 = np.float32
@@ -222,3 +223,39 @@ def test_instance_bounds_match_oracle_and_contain_vertices():
         lo = np.array([p[i]["min_x"], p[i]["min_y"], p[i]["min_z"]])
         hi = np.array([p[i]["max_x"], p[i]["max_y"], p[i]["max_z"]])
         assert (w >= lo - 1e-4).all() and (w <= hi + 1e-4).all()
+
+
+def test_refit_of_unchanged_vertices_reproduces_the_builder_and_is_tight():
+    """Refit (oracle) with the vertices the BLAS was built from gives the
+    builder's nodes byte for byte; after moving the vertices every node box
+    still holds all of its triangles' vertices and touches them on all six
+    faces (before inflation)."""
+    mesh = scenes.mesh_torus(24, 12, 3.0, 1.0)
+    builder = scenes.Builder()
+    verts, idx, blas, ranges, local = scenes.mesh_library([mesh], builder)
+    vb, ib, tc, bb = ranges[0]
+    again = oracle.refit_blas(blas, verts, idx, vb, ib)
+    assert again.tobytes() == blas.tobytes()
+    moved = verts.copy()
+    rng = np.random.default_rng(5)
+    for k in ("px", "py", "pz"):
+        moved[k] = (moved[k] * np.float32(0.8) + rng.normal(0, 0.05, len(moved)).astype(np.float32)).astype(np.float32)
+    ref = oracle.refit_blas(blas, moved, idx, vb, ib)
+    P = np.stack([moved["px"], moved["py"], moved["pz"]], 1)
+
+    def tris_of(i):
+        n = ref[i]
+        if n["count"] > 0:
+            return list(range(n["left"], n["left"] + n["count"]))
+        return tris_of(n["left"]) + tris_of(n["right"])
+
+    for i in range(len(ref)):
+        t = np.array(tris_of(i))
+        pts = P[idx.reshape(-1, 3)[t].reshape(-1)]
+        lo = np.array([ref[i]["min_x"], ref[i]["min_y"], ref[i]["min_z"]])
+        hi = np.array([ref[i]["max_x"], ref[i]["max_y"], ref[i]["max_z"]])
+        assert np.all(pts >= lo) and np.all(pts <= hi)
+        ext = pts.max(0) - pts.min(0)
+        for a in range(3):
+            if ext[a] >= 1e-4:
+                assert lo[a] == pts[:, a].min() and hi[a] == pts[:, a].max()
