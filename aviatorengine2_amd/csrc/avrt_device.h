@@ -125,6 +125,24 @@ __device__ __forceinline__ float pow32(float x)
     return y * y;
 }
 
+// Band order of the pixels of a W x H image: bands of 8 rows (the last may
+// be shorter), column by column within a band, top to bottom within a
+// column. band_pixel inverts band_index.
+__host__ __device__ __forceinline__ uint32_t band_index(uint32_t x, uint32_t y, uint32_t W, uint32_t H)
+{
+    const uint32_t b = y >> 3;
+    const uint32_t bh = (H - (b << 3)) < 8u ? (H - (b << 3)) : 8u;
+    return b * 8u * W + x * bh + (y - (b << 3));
+}
+__host__ __device__ __forceinline__ void band_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t *x, uint32_t *y)
+{
+    const uint32_t b = q / (8u * W);
+    const uint32_t r = q - b * 8u * W;
+    const uint32_t bh = (H - (b << 3)) < 8u ? (H - (b << 3)) : 8u;
+    *x = r / bh;
+    *y = (b << 3) + (r - *x * bh);
+}
+
 struct Rng {
     uint32_t s;
     __device__ __forceinline__ float next()
@@ -336,6 +354,16 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
 // child first), so hits at exactly equal t resolve as there; false visits the
 // nearer child first.
 //
+// node_visit() / leaf_phase() split the same walk into the two phases of a
+// speculative while-while traversal (Aila and Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009): a lane that reaches a leaf
+// postpones it in `leaf` and keeps visiting interior nodes of the same tree
+// until it reaches a second leaf, so a wave runs its node visits and its leaf
+// tests as separate, fuller passes. Leaves are tested in the order step()
+// tests them; interior nodes visited while a leaf waits are culled against
+// the `far` from before that leaf, which can add visits but does not change
+// the closest hit.
+//
 // FILTER restricts the objects considered:
 //   kAll        every object, closest hit;
 //   kLightGlass only lights and glass (kGeomLG), closest hit;
@@ -348,14 +376,21 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
 // requested class.
 constexpr int kAll = 0, kLightGlass = 1, kOpaqueAny = 2, kRaster = 3, kNonRaster = 4;
 
-template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll>
+// Outcome of a leaf visit.
+constexpr int kVisitNext = 0;  // leaf done, continue with the next entry
+constexpr int kVisitEnter = 1; // entered an instance's BLAS, ref is its root
+constexpr int kVisitStop = 2;  // any-hit walk found its hit
+
+template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll, int LQ = 1>
 struct Traversal {
     F3 o, d, winv;    // world ray
     F3 ao, ainv;      // active ray (world, or mesh-local inside a BLAS)
     F3 ad;
     float time, near, far;
     float term; // kOpaqueAny: a hit closer than this ends the walk
-    uint32_t ref;
+    uint32_t ref;  // next entry to visit; kRefInvalid once the stack ran empty
+    uint32_t leaf;  // oldest postponed leaf of the speculative walk, kRefInvalid if none
+    uint32_t leaf2; // the next one (LQ = 2)
     uint32_t cur_obj, tri_base;
     bool in_blas;
     uint32_t steps;
@@ -385,16 +420,30 @@ struct Traversal {
         cur_obj = 0;
         tri_base = 0;
         ref = 0; // TLAS super-root
+        leaf = kRefInvalid;
+        leaf2 = kRefInvalid;
         steps = 0;
         return sc.tlas != nullptr;
     }
 
-    // Pops the next entry; false when the stack is empty (ray finished).
+    // Pops the next entry into ref; false (and ref = kRefInvalid) when the
+    // stack is empty.
     __device__ __forceinline__ bool next()
     {
-        if (st.empty()) return false;
+        if (st.empty()) {
+            ref = kRefInvalid;
+            return false;
+        }
         ref = st.pop();
         return true;
+    }
+
+    __device__ __forceinline__ void leave_blas()
+    {
+        in_blas = false;
+        ao = o;
+        ad = d;
+        ainv = winv;
     }
 
     __device__ __forceinline__ bool step(const SceneDev &sc)
@@ -403,13 +452,87 @@ struct Traversal {
         // entry; the cap only bounds the loop should a malformed tree reach it.
         if (++steps >= kMaxTraversalSteps) return false;
         if (ref == kRefReturn) {
-            in_blas = false;
-            ao = o;
-            ad = d;
-            ainv = winv;
+            leave_blas();
             return next();
         }
-        if ((ref & kLeafFlag) == 0) {
+        if ((ref & kLeafFlag) == 0) return interior(sc);
+        const int r = leaf_visit(sc, ref, kRefInvalid, kRefInvalid);
+        if (r == kVisitEnter) return true;
+        if (r == kVisitStop) return false;
+        return next();
+    }
+
+    // ---- speculative while-while walk (nearest-first) ----
+    // Moves leaves in ref into the free postponed slots (oldest in `leaf`),
+    // popping the next entry after each.
+    __device__ __forceinline__ static bool is_leaf(uint32_t r)
+    {
+        return (r & kLeafFlag) != 0 && r != kRefReturn && r != kRefInvalid;
+    }
+    __device__ __forceinline__ void postpone()
+    {
+        if (leaf == kRefInvalid && is_leaf(ref)) {
+            leaf = ref;
+            next();
+        }
+        if (LQ > 1 && leaf2 == kRefInvalid && leaf != kRefInvalid && is_leaf(ref)) {
+            leaf2 = ref;
+            next();
+        }
+    }
+    // An interior node to visit, or a BLAS to leave with no leaf of it waiting.
+    __device__ __forceinline__ bool node_ready() const
+    {
+        return (ref & kLeafFlag) == 0 || (ref == kRefReturn && leaf == kRefInvalid);
+    }
+    __device__ __forceinline__ bool finished() const { return ref == kRefInvalid && leaf == kRefInvalid; }
+
+    // One visit of the node phase; only when node_ready().
+    __device__ __forceinline__ void node_visit(const SceneDev &sc)
+    {
+        if (++steps >= kMaxTraversalSteps) {
+            ref = kRefInvalid;
+            leaf = kRefInvalid;
+            leaf2 = kRefInvalid;
+            return;
+        }
+        if (ref == kRefReturn) {
+            leave_blas();
+            next();
+        } else {
+            interior(sc);
+        }
+        postpone();
+    }
+
+    // Tests the oldest postponed leaf; only when leaf != kRefInvalid.
+    // Entering an instance saves the speculative ref and the other postponed
+    // leaf on the stack beneath the rest of the leaf, so the walk resumes with
+    // them after the BLAS.
+    __device__ __forceinline__ void leaf_phase(const SceneDev &sc)
+    {
+        const uint32_t lref = leaf;
+        leaf = leaf2;
+        leaf2 = kRefInvalid;
+        ++steps;
+        const int r = leaf_visit(sc, lref, ref, leaf);
+        if (r == kVisitStop) {
+            ref = kRefInvalid;
+            leaf = kRefInvalid;
+            return;
+        }
+        if (r == kVisitEnter) {
+            leaf = kRefInvalid;
+            return;
+        }
+        postpone();
+    }
+
+    // Interior node `ref`: tests the four children and moves ref to the next
+    // entry (a child, else the top of the stack); false when none is left.
+    __device__ __forceinline__ bool interior(const SceneDev &sc)
+    {
+        {
             const Node4 *np = (in_blas ? sc.blas : sc.tlas) + ref;
             const float4 LX = *reinterpret_cast<const float4 *>(np->lox);
             const float4 HX = *reinterpret_cast<const float4 *>(np->hix);
@@ -478,8 +601,15 @@ struct Traversal {
             ref = r0;
             return true;
         }
-        const uint32_t first = ref & 0x00FFFFFFu;
-        const uint32_t count = (ref >> 24) & 0x7Fu;
+    }
+
+    // Leaf `lref`: tests its triangles (in a BLAS) or objects. Entering an
+    // instance pushes `pending` then `pending2` (each when valid), the rest of
+    // the leaf and the return marker, and sets ref to the BLAS root.
+    __device__ __forceinline__ int leaf_visit(const SceneDev &sc, uint32_t lref, uint32_t pending, uint32_t pending2)
+    {
+        const uint32_t first = lref & 0x00FFFFFFu;
+        const uint32_t count = (lref >> 24) & 0x7Fu;
         if (in_blas) {
             for (uint32_t i = 0; i < count; i++) {
                 const TriPre *tp = sc.tris + tri_base + first + i;
@@ -505,9 +635,9 @@ struct Traversal {
                 hit.prim = first + i;
                 hit.alpha = al;
                 hit.beta = be;
-                if (FILTER == kOpaqueAny && t < term) return false;
+                if (FILTER == kOpaqueAny && t < term) return kVisitStop;
             }
-            return next();
+            return kVisitNext;
         }
         for (uint32_t i = 0; i < count; i++) {
             const uint32_t slot = first + i;
@@ -529,6 +659,8 @@ struct Traversal {
             if (shape == AVRT_SHAPE_MESH) {
                 // Leave the rest of this leaf for later and walk the
                 // instance's BLAS in mesh-local space (ToLocal).
+                if (pending != kRefInvalid) st.push(pending);
+                if (pending2 != kRefInvalid) st.push(pending2);
                 if (i + 1 < count) st.push(make_leaf(slot + 1, count - i - 1));
                 st.push(kRefReturn);
                 const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];
@@ -544,7 +676,7 @@ struct Traversal {
                 tri_base = __float_as_uint(g.c[1]);
                 in_blas = true;
                 ref = __float_as_uint(g.c[0]);
-                return true;
+                return kVisitEnter;
             } else if (shape == AVRT_SHAPE_SPHERE) {
                 h = test_sphere(g, o, d, time, near, far, &t);
             } else if (shape == AVRT_SHAPE_TRIANGLE) {
@@ -561,10 +693,10 @@ struct Traversal {
                 hit.prim = ~0u;
                 hit.alpha = al;
                 hit.beta = be;
-                if (FILTER == kOpaqueAny && t < term) return false;
+                if (FILTER == kOpaqueAny && t < term) return kVisitStop;
             }
         }
-        return next();
+        return kVisitNext;
     }
 };
 

@@ -41,12 +41,12 @@ using namespace avrt::dev;
 
 namespace {
 
-__global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t first_sample, uint32_t n, PathBufs out,
-                                                   float4 *Lpath)
+__global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t first_sample, uint32_t ns, uint32_t n,
+                                                   PathBufs out, float4 *Lpath)
 {
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= n) return;
-    raygen_path(fp, first_sample, pid, pid, out, Lpath);
+    raygen_path(fp, first_sample, ns, pid, pid, out, Lpath);
 }
 
 template <int MODE, bool WAVE_NEE>
@@ -325,6 +325,7 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     *out_ctx = nullptr;
     AvrtCreateOptions o{};
     if (opts) o = *opts;
+    if (o.trace_schedule > 2) return set_err(nullptr, AVRT_E_INVALID_ARG, "trace_schedule must be 0, 1 or 2");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(nullptr, AVRT_E_DEVICE, "no HIP device");
     if (o.device < 0 || o.device >= ndev) return set_err(nullptr, AVRT_E_INVALID_ARG, "device ordinal out of range");
@@ -348,6 +349,9 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
     c->trace_blocks_per_cu = nb;
+    nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_spec<false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
+    c->spec_blocks_per_cu = nb;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess || hybrid_init(c, lut) != AVRT_OK ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
@@ -643,8 +647,14 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
     return std::max(1u, std::min(blocks_for(n), resident));
 }
 
+// True when the shadow-ray launch resolves glass continuations and occlusion
+// itself (the speculative persistent walk).
+static bool fused_shadow(const avrt_ctx *c) { return c->opts.trace_schedule == 0 && !c->ref_order; }
+
+// Closest-hit traversal of a ray queue into hitA/hitB; with fused_shadow(),
+// a shadow queue is resolved in place instead (contrib, segs).
 static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n, const float4 *ro, const float4 *rd,
-                        const float4 *rlp)
+                        const float4 *rlp, float4 *contrib = nullptr, uint32_t *segs = nullptr)
 {
     uint4 *hA = static_cast<uint4 *>(c->hitA.p);
     float *hB = static_cast<float *>(c->hitB.p);
@@ -667,6 +677,16 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
     if (c->ref_order) {
         if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
         else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+    } else if (c->opts.trace_schedule == 0) {
+        const dim3 sgrid(std::max(1u, std::min(blocks_for(n), static_cast<uint32_t>(c->cu_count * c->spec_blocks_per_cu))));
+        if (shadow) {
+            if (contrib == nullptr || segs == nullptr) return set_err(c, AVRT_E_STATE, "shadow launch without contribution slots");
+            hipLaunchKernelGGL((k_trace_spec<true>), sgrid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work, contrib,
+                               segs);
+        } else {
+            hipLaunchKernelGGL((k_trace_spec<false>), sgrid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work,
+                               static_cast<float4 *>(nullptr), static_cast<uint32_t *>(nullptr));
+        }
     } else {
         if (shadow) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
         else hipLaunchKernelGGL((k_trace<false, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
@@ -702,7 +722,7 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
     const bool wave_nee = (MODE == 1) && fp.num_lights > 0 && fp.num_lights <= kWaveLights;
     float4 *Lpath = static_cast<float4 *>(c->Lpath.p);
     HIPX(c, hipEventRecord(c->ev_a, c->stream));
-    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, fp, s0, n, buf[0], Lpath);
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, fp, s0, ns, n, buf[0], Lpath);
     HIPX(c, hipGetLastError());
     HIPX(c, hipEventRecord(c->ev_b, c->stream));
     HIPX(c, hipEventSynchronize(c->ev_b));
@@ -737,6 +757,15 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
             uint32_t nsr = hc[2];
             int q = 0;
             HIPX(c, hipEventRecord(c->ev_a, c->stream));
+            if (fused_shadow(c) && nsr > 0) {
+                // one launch: glass continuations stay in their lanes; segments counted in counters[3]
+                c->stats.rays_shadow += nsr;
+                if ((rc = launch_trace(c, sc, true, nsr, sr[0].o, sr[0].d, sr[0].lp, nee.contrib, counters + 3))) return rc;
+                uint32_t h2[4];
+                if ((rc = read_counters(c, h2))) return rc;
+                c->stats.rays_shadow += h2[3];
+                nsr = 0;
+            }
             for (int step = 0; step < 4 && nsr > 0; step++) {
                 c->stats.rays_shadow += nsr;
                 if ((rc = launch_trace(c, sc, true, nsr, sr[q].o, sr[q].d, sr[q].lp))) return rc;
@@ -893,7 +922,7 @@ int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object,
     rc = ensure_paths(c, P, 0);
     if (rc) return rc;
     PathBufs buf{static_cast<float4 *>(c->so[0].p), static_cast<float4 *>(c->sd[0].p), static_cast<float4 *>(c->st[0].p)};
-    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, 0u, P, buf,
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, 0u, 1u, P, buf,
                        static_cast<float4 *>(c->Lpath.p));
     HIPX(c, hipGetLastError());
     const SceneDev sc = scene_dev(c);
@@ -902,12 +931,14 @@ int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object,
     std::vector<uint4> h(P);
     HIPX(c, hipMemcpyAsync(h.data(), c->hitA.p, sizeof(uint4) * P, hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
+    // rays are in band order (one sample per pixel); outputs in raster order
     for (uint32_t p = 0; p < P; p++) {
+        const uint4 &r = h[band_index(p % cfg->width, p / cfg->width, cfg->width, cfg->height)];
         float t;
-        std::memcpy(&t, &h[p].x, 4);
-        const bool hit = h[p].y != ~0u;
-        if (hit_object) hit_object[p] = h[p].y;
-        if (hit_prim) hit_prim[p] = hit ? h[p].z : ~0u;
+        std::memcpy(&t, &r.x, 4);
+        const bool hit = r.y != ~0u;
+        if (hit_object) hit_object[p] = r.y;
+        if (hit_prim) hit_prim[p] = hit ? r.z : ~0u;
         if (hit_t) hit_t[p] = hit ? t : 0.0f;
     }
     return AVRT_OK;

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_raygen(SegLaunch L, const JobDev
     const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
     if (local >= L.count[j]) return;
     const JobDev &J = jobs[j];
-    raygen_path(J.fp, J.first_sample, local, J.path_base + local, out, Lpath);
+    raygen_path(J.fp, J.first_sample, J.ns, local, J.path_base + local, out, Lpath);
 }
 
 // Closest-hit queries of the jobs' queues (extension rays, or shadow rays

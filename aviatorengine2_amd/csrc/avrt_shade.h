@@ -415,6 +415,123 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
     }
 }
 
+// Closest-hit traversal (nearest child first) by persistent lanes with the
+// speculative while-while walk of Traversal, scheduled per wave step: the
+// wave runs the pass that has the most lanes ready, either one interior node
+// per lane that can advance (node_ready) or one postponed leaf per lane that
+// holds one, BLAS and TLAS leaves in separate passes (their code differs).
+// Lanes whose ray is finished write its hit and, once kRefillMin of them are
+// free (or nothing else can run), claim new rays (one atomic per wave).
+// Every pass advances at least one lane, and every ray's walk is finite, so
+// the loop ends once the queue is drained and every lane is done.
+//
+// SHADOW rays (the queue of ShadowRays) are resolved in place, with the
+// operations of shadow_resolve_ray: a hit on an opaque object before the
+// light point zeroes the ray's contribution slot; a hit on glass continues
+// the ray from just past the hit point, in the same lane, up to 4 segments
+// in all; anything else leaves the contribution. Continuation segments are
+// added to *segs (one atomic per wave).
+constexpr int kRefillMin = 8;
+
+template <bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace_spec(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                       const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                       uint4 *__restrict__ hitA, float *__restrict__ hitB,
+                                                       uint32_t *work, float4 *__restrict__ contrib, uint32_t *segs)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    uint32_t spill_mem[kStackSpill];
+    Traversal<false, kStackLds, kBlock, kStackSpill> tr;
+    uint32_t idx = 0;
+    uint32_t slot = 0, step = 0, nseg = 0; // SHADOW: contribution slot, glass steps taken, continuations
+    bool have = false, done = false;
+    while (true) {
+        if (have && tr.finished()) {
+            if (SHADOW) {
+                const float t = tr.hit.t;
+                const F3 so = tr.o, dir = tr.d;
+                const float4 L = rlp[idx];
+                const F3 lp = f3(L.x, L.y, L.z);
+                const float bound = length(lp - so) - 0.001f;
+                bool more = false;
+                if (tr.hit.obj != ~0u && !(t >= bound)) {
+                    const uint32_t ct = sc.objects[tr.hit.obj].colorType;
+                    if (ct != AVRT_DIFFUSE_LIGHT) {
+                        if (ct != AVRT_DIELECTRIC) {
+                            contrib[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        } else if (step + 1u < 4u) {
+                            const F3 no = (so + dir * t) + (dir * 0.001f);
+                            step = step + 1u;
+                            nseg++;
+                            more = tr.init(sc, no, dir, tr.time, 0.001f, length(lp - no) - 0.001f, lds_stack, spill_mem);
+                        }
+                    }
+                }
+                have = more;
+            } else {
+                hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
+                hitB[idx] = tr.hit.beta;
+                have = false;
+            }
+        }
+        const bool can_node = have && tr.node_ready();
+        const bool leaf_b = have && tr.leaf != kRefInvalid && tr.in_blas;
+        const bool leaf_t = have && tr.leaf != kRefInvalid && !tr.in_blas;
+        const uint64_t m_need = __ballot(!have && !done);
+        const uint64_t m_node = __ballot(can_node);
+        const uint64_t m_lb = __ballot(leaf_b);
+        const uint64_t m_lt = __ballot(leaf_t);
+        const bool idle = (m_node | m_lb | m_lt) == 0ull;
+        if (m_need != 0ull && (idle || __popcll(m_need) >= kRefillMin)) {
+            const bool need = !have && !done;
+            const uint32_t cnt = static_cast<uint32_t>(__popcll(m_need));
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m_need >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m_need), 0u));
+            const int leader = __ffsll(static_cast<unsigned long long>(m_need)) - 1;
+            uint32_t base = 0;
+            if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, leader);
+            const uint32_t avail = base < n ? min(cnt, n - base) : 0u;
+            if (need && rank < avail) {
+                idx = base + rank;
+                {
+                    const float4 O = ro[idx];
+                    const float4 D = rd[idx];
+                    const F3 o = f3(O.x, O.y, O.z);
+                    float far = 1000000.0f;
+                    if (SHADOW) {
+                        const float4 L = rlp[idx];
+                        far = length(f3(L.x, L.y, L.z) - o) - 0.001f;
+                        slot = __float_as_uint(D.w);
+                        step = __float_as_uint(L.w);
+                    }
+                    have = tr.init(sc, o, f3(D.x, D.y, D.z), O.w, 0.001f, far, lds_stack, spill_mem);
+                    if (!have && !SHADOW) {
+                        hitA[idx] = make_uint4(__float_as_uint(far), ~0u, ~0u, 0u);
+                        hitB[idx] = 0.0f;
+                    }
+                }
+            } else if (need && avail == 0) {
+                done = true; // the queue is drained
+            }
+            continue;
+        }
+        if (idle) break; // nothing to claim and no lane holds a ray
+        const int nn = __popcll(m_node), nb = __popcll(m_lb), nt = __popcll(m_lt);
+        if (nn > 0 && nn >= nb + nt) {
+            if (can_node) tr.node_visit(sc);
+        } else if (nb >= nt) {
+            if (leaf_b) tr.leaf_phase(sc);
+        } else {
+            if (leaf_t) tr.leaf_phase(sc);
+        }
+    }
+    if (SHADOW) {
+        const uint32_t total = wave_sum(nseg);
+        if (__lane_id() == 0 && total != 0u) atomicAdd(segs, total);
+    }
+}
+
 __device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)
 {
     float4 L = Lpath[pid];
@@ -472,16 +589,21 @@ __device__ __forceinline__ float round_half(float x) { return __half2float(__flo
 // ---------------------------------------------------------------------------
 // Stage bodies shared by the plain and the batched (served) kernels
 // ---------------------------------------------------------------------------
-// Camera path `local` (= sample * P + pixel) of a chunk of sample frames
-// starting at `first_sample`, stored at queue position / path id `pid`
+// Camera path `local` of a chunk of `ns` sample frames starting at
+// `first_sample`, stored at queue position / path id `pid`
 // (ray_trace.comp.hybrid.hlsl:1346-1373, path_trace.comp.hlsl:789-811).
-__device__ __forceinline__ void raygen_path(const FrameParams &fp, uint32_t first_sample, uint32_t local, uint32_t pid,
-                                            PathBufs out, float4 *Lpath)
+// Paths are ordered pixel-major (local = band_index(x, y) * ns + sample), so
+// a wave's 64 camera rays cover a compact patch of the image and a
+// contiguous range of paths a band of it; each path's RNG stream depends on
+// its pixel and sample only, and resolve_pixel sums a pixel's samples in
+// sample order, so the order changes no result.
+__device__ __forceinline__ void raygen_path(const FrameParams &fp, uint32_t first_sample, uint32_t ns, uint32_t local,
+                                            uint32_t pid, PathBufs out, float4 *Lpath)
 {
-    const uint32_t P = fp.width * fp.height;
-    const uint32_t sidx = local / P;
-    const uint32_t p = local - sidx * P;
-    const uint32_t x = p % fp.width, y = p / fp.width;
+    const uint32_t q = local / ns;
+    const uint32_t sidx = local - q * ns;
+    uint32_t x, y;
+    band_pixel(q, fp.width, fp.height, &x, &y);
     const uint32_t batch = fp.batch + first_sample + sidx;
     Rng rng{x + y * fp.width + batch * fp.width * fp.height + 1u};
     const float jx = rng.next() - 0.5f;
@@ -720,11 +842,12 @@ template <int MODE>
 __device__ __forceinline__ void resolve_pixel(const FrameParams &fp, uint32_t p, uint32_t first_sample, uint32_t ns,
                                               const float4 *Lpath, float4 *image, uchar4 *display)
 {
-    const uint32_t P = fp.width * fp.height;
+    const uint32_t x = p % fp.width, y = p / fp.width;
+    const size_t q = static_cast<size_t>(band_index(x, y, fp.width, fp.height)) * ns;
     float4 acc = image[p];
     for (uint32_t s = 0; s < ns; s++) {
         const uint32_t batch = fp.batch + first_sample + s;
-        const float4 c = Lpath[static_cast<size_t>(s) * P + p];
+        const float4 c = Lpath[q + s];
         if (MODE == 0) {
             const float px = batch > 0 ? acc.x : 0.0f;
             const float py = batch > 0 ? acc.y : 0.0f;

@@ -30,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# extension-ray traversal kernel of each trace schedule (nearest-first)
+TRACE_KERNEL = {0: "k_trace_spec<false>", 1: "k_trace_lane<false, false>", 2: "k_trace<false, false>"}
 
 
 def log(msg):
@@ -54,7 +56,7 @@ def measured_traffic(args):
     by the copy-kernel calibration factors), when this run is that workload."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     default = (args.width, args.height, args.spp, args.depth, args.instances) == (1920, 1080, 8, 8, 1024)
-    if not default or args.lane_trace or not os.path.exists(path):
+    if not default or args.lane_trace or args.trace_schedule not in (None, 0) or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
@@ -76,6 +78,8 @@ def main():
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lane-trace", action="store_true", help="one ray per lane instead of persistent lanes")
+    ap.add_argument("--trace-schedule", type=int, default=None,
+                    help="0 speculative persistent walk (default), 1 one ray per lane, 2 one-step persistent walk")
     ap.add_argument("--served-jobs", type=int, default=160, help="jobs of the served stream (0 skips the leg)")
     ap.add_argument("--served-load", type=float, default=0.6,
                     help="Poisson arrival rate of the latency run, as a fraction of the measured served capacity")
@@ -101,7 +105,8 @@ def main():
     log(f"scene built in {time.perf_counter() - t0:.1f}s: {len(packed.objects)} objects, "
         f"{len(packed.indices) // 3} BLAS triangles, {len(packed.blas)} BLAS nodes")
 
-    r = Renderer().Initialize(device=0, lane_trace=args.lane_trace)
+    sched = args.trace_schedule if args.trace_schedule is not None else (1 if args.lane_trace else 0)
+    r = Renderer().Initialize(device=0, trace_schedule=sched)
     r.UploadMeshGeometry(library[0], library[1], library[2])
     r.UploadTextures(spec.texture)
     r.SetAuthoring(objs, inst, local)
@@ -149,7 +154,7 @@ def main():
         achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                    "kernel": "k_trace<false, false>" if not args.lane_trace else "k_trace_lane<false, false>",
+                    "kernel": TRACE_KERNEL[sched],
                     "bytes_per_ray": b_ext, "avg_launch_ms": per_launch_ms}
 
     served = run_served(args, mrays) if args.served_jobs > 0 else None

@@ -84,6 +84,16 @@ def test_create_without_gpu_fails_cleanly_or_succeeds(avrt_lib):
     assert b"device" in avrt_lib.avrt_last_error(None)
 
 
+def test_create_rejects_unknown_trace_schedule(avrt_lib):
+    """Options are checked before any device call."""
+    o = abi.AvrtCreateOptions()
+    o.trace_schedule = 3
+    ctx = ctypes.c_void_p()
+    assert avrt_lib.avrt_create(ctypes.byref(o), ctypes.byref(ctx)) == -1
+    assert b"trace_schedule" in avrt_lib.avrt_last_error(None)
+    assert not ctx.value
+
+
 def test_version_string(avrt_lib):
     assert avrt_lib.avrt_version().startswith(b"avrt")
 

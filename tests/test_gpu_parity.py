@@ -225,17 +225,23 @@ def test_chunked_paths_equal_single_wave(c2):
     assert np.array_equal(a, b)
 
 
-def test_trace_schedules_agree(c3):
-    """Persistent lanes (default) and one ray per lane give the same image."""
-    say("persistent vs lane traversal schedule")
+@pytest.mark.parametrize("ref_order", [False, True])
+def test_trace_schedules_agree(c3, ref_order):
+    """The speculative persistent walk (default), one ray per lane and the
+    one-step persistent walk give the same image, and the same primary hits."""
+    say(f"traversal schedules 0/1/2, ref_order={ref_order}")
     cfg = small_cfg(c3.config, 160, 90, 2)
-    out = []
-    for lane in (False, True):
-        with Renderer().Initialize(lane_trace=lane) as r:
+    out, hits = [], []
+    for sched in (0, 1, 2):
+        with Renderer().Initialize(trace_schedule=sched, reference_order=ref_order) as r:
             r.LoadScene(c3)
             r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
             out.append(r.ReadImage())
-    assert np.array_equal(out[0], out[1])
+            hits.append(r.TracePrimary(cfg))
+    for k in (1, 2):
+        assert np.array_equal(out[0], out[k]), k
+        for a, b in zip(hits[0], hits[k]):
+            assert np.array_equal(a, b), k
 
 
 def test_empty_scene_renders_sky():
