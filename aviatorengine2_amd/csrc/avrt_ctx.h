@@ -128,6 +128,9 @@ struct avrt_ctx {
 
     AvrtStats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_a = nullptr, ev_b = nullptr;
+    // second stream: bounce b's shadow walk overlapping bounce b + 1's extension walk
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_shaded = nullptr, ev_nee = nullptr;
 };
 
 namespace avrt {

@@ -334,7 +334,9 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     c->device = o.device;
     c->opts = o;
     c->ref_order = (o.traversal_order != 0);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return set_err(nullptr, AVRT_E_DEVICE, "hipStreamCreate failed");
     }
@@ -354,7 +356,10 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     c->spec_blocks_per_cu = nb;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess || hybrid_init(c, lut) != AVRT_OK ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess) {
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
+        hipEventCreate(&c->ev_s0) != hipSuccess || hipEventCreate(&c->ev_s1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_shaded, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_nee, hipEventDisableTiming) != hipSuccess) {
         avrt_destroy(c);
         return set_err(nullptr, AVRT_E_DEVICE, "device initialisation failed");
     }
@@ -367,6 +372,7 @@ int avrt_destroy(avrt_ctx *c)
     if (c == nullptr) return AVRT_E_INVALID_ARG;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     DevBuf *bufs[] = {&c->d_blas, &c->d_blas2_src, &c->d_refit_node, &c->d_refit_tree, &c->d_refit_box,
                       &c->d_verts, &c->d_indices, &c->d_blas2, &c->d_tris, &c->d_objects, &c->d_geom, &c->d_tlas2,
                       &c->d_lights, &c->d_instances, &c->d_tex, &c->d_image, &c->d_display, &c->so[0], &c->so[1],
@@ -377,9 +383,10 @@ int avrt_destroy(avrt_ctx *c)
                       &c->srl[1], &c->nee_tp, &c->nee_contrib};
     for (DevBuf *b : bufs) release(*b);
     hybrid_release(c);
-    for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b})
+    for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b, c->ev_s0, c->ev_s1, c->ev_shaded, c->ev_nee})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     delete c;
     return AVRT_OK;
 }
@@ -587,7 +594,7 @@ static int ensure_paths(avrt_ctx *c, size_t n, uint32_t nlights)
         if ((rc = ensure(c, c->sha, n * 16))) return rc;
         if ((rc = ensure(c, c->shw, n * 16))) return rc;
         if ((rc = ensure(c, c->nee_tp, n * 16))) return rc;
-        if ((rc = ensure(c, c->counters, 64))) return rc;
+        if ((rc = ensure(c, c->counters, 256))) return rc;
         c->cap_paths = n;
     }
     const size_t ns = n * (nlights <= kWaveLights ? nlights : 0);
@@ -652,52 +659,58 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
 static bool fused_shadow(const avrt_ctx *c) { return c->opts.trace_schedule == 0 && !c->ref_order; }
 
 // Closest-hit traversal of a ray queue into hitA/hitB; with fused_shadow(),
-// a shadow queue is resolved in place instead (contrib, segs).
+// a shadow queue is resolved in place instead (contrib, segs). Launched on
+// `st` (the context's stream when null); each stream has its own claim counter.
 static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n, const float4 *ro, const float4 *rd,
-                        const float4 *rlp, float4 *contrib = nullptr, uint32_t *segs = nullptr)
+                        const float4 *rlp, float4 *contrib = nullptr, uint32_t *segs = nullptr, hipStream_t st = nullptr)
 {
+    if (st == nullptr) st = c->stream;
     uint4 *hA = static_cast<uint4 *>(c->hitA.p);
     float *hB = static_cast<float *>(c->hitB.p);
     const dim3 block(kBlock);
     if (c->opts.trace_schedule == 1) {
         const dim3 grid(blocks_for(n));
         if (c->ref_order) {
-            if (shadow) hipLaunchKernelGGL((k_trace_lane<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
-            else hipLaunchKernelGGL((k_trace_lane<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+            if (shadow) hipLaunchKernelGGL((k_trace_lane<true, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB);
+            else hipLaunchKernelGGL((k_trace_lane<true, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB);
         } else {
-            if (shadow) hipLaunchKernelGGL((k_trace_lane<false, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
-            else hipLaunchKernelGGL((k_trace_lane<false, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB);
+            if (shadow) hipLaunchKernelGGL((k_trace_lane<false, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB);
+            else hipLaunchKernelGGL((k_trace_lane<false, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB);
         }
         HIPX(c, hipGetLastError());
         return AVRT_OK;
     }
-    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + 8;
-    HIPX(c, hipMemsetAsync(work, 0, 4, c->stream));
+    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + (st == c->stream2 ? 9 : 8);
+    HIPX(c, hipMemsetAsync(work, 0, 4, st));
     const dim3 grid(trace_grid(c, n));
     if (c->ref_order) {
-        if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
-        else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+        if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
+        else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
     } else if (c->opts.trace_schedule == 0) {
         const dim3 sgrid(std::max(1u, std::min(blocks_for(n), static_cast<uint32_t>(c->cu_count * c->spec_blocks_per_cu))));
         if (shadow) {
             if (contrib == nullptr || segs == nullptr) return set_err(c, AVRT_E_STATE, "shadow launch without contribution slots");
-            hipLaunchKernelGGL((k_trace_spec<true>), sgrid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work, contrib,
-                               segs);
+            hipLaunchKernelGGL((k_trace_spec<true>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs);
         } else {
-            hipLaunchKernelGGL((k_trace_spec<false>), sgrid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work,
+            hipLaunchKernelGGL((k_trace_spec<false>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work,
                                static_cast<float4 *>(nullptr), static_cast<uint32_t *>(nullptr));
         }
     } else {
-        if (shadow) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
-        else hipLaunchKernelGGL((k_trace<false, false>), grid, block, 0, c->stream, sc, n, ro, rd, rlp, hA, hB, work);
+        if (shadow) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
+        else hipLaunchKernelGGL((k_trace<false, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
     }
     HIPX(c, hipGetLastError());
     return AVRT_OK;
 }
 
-static int read_counters(avrt_ctx *c, uint32_t *host4)
+// Bounce counters: slot b & 1 of bounce b holds {next queue size, NEE records,
+// shadow rays, shadow continuation segments}; read_counters copies both slots.
+constexpr uint32_t kSlotBase = 16;
+
+static int read_counters(avrt_ctx *c, uint32_t *host8)
 {
-    HIPX(c, hipMemcpyAsync(host4, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipMemcpyAsync(host8, static_cast<uint32_t *>(c->counters.p) + kSlotBase, 32, hipMemcpyDeviceToHost,
+                           c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
     return AVRT_OK;
 }
@@ -729,53 +742,76 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
     c->stats.ms_other += elapsed(c->ev_a, c->ev_b);
     c->stats.paths += n;
     int cur = 0;
+    // With the fused shadow walk, bounce b's shadow rays are traced on the
+    // second stream while the first stream traces bounce b + 1's extension
+    // rays; the shade of bounce b + 1 (and the resolve) waits for bounce b's
+    // NEE sums, so each path's radiance receives its additions in the same
+    // order as without the overlap.
+    const bool overlap = MODE == 1 && wave_nee && fused_shadow(c);
+    bool shadow_pending = false; // a shadow walk on stream2 whose counters are not read yet
+    uint32_t pend_slot = 0;
     for (uint32_t b = 0; b < fp.depth && n > 0; b++) {
+        const uint32_t slot = b & 1u;
+        uint32_t *cnt = counters + kSlotBase + 4 * slot;
         if (b == 0) c->stats.rays_primary += n;
         else c->stats.rays_bounce += n;
-        HIPX(c, hipMemsetAsync(counters, 0, 16, c->stream));
+        HIPX(c, hipMemsetAsync(cnt, 0, 16, c->stream));
         HIPX(c, hipEventRecord(c->ev_a, c->stream));
         int rc = launch_trace(c, sc, false, n, buf[cur].o, buf[cur].d, nullptr);
         if (rc) return rc;
         HIPX(c, hipEventRecord(c->ev_b, c->stream));
         c->stats.launches_extend++;
+        if (shadow_pending) HIPX(c, hipStreamWaitEvent(c->stream, c->ev_nee, 0));
         if (wave_nee)
             hipLaunchKernelGGL((k_shade<MODE, true>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
                                static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
-                               sr[0], nee, Lpath, counters);
+                               sr[0], nee, Lpath, cnt);
         else
             hipLaunchKernelGGL((k_shade<MODE, false>), dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, fp, n, b, buf[cur],
                                static_cast<const uint4 *>(c->hitA.p), static_cast<const float *>(c->hitB.p), buf[cur ^ 1], sh,
-                               sr[0], nee, Lpath, counters);
+                               sr[0], nee, Lpath, cnt);
         HIPX(c, hipGetLastError());
         HIPX(c, hipEventRecord(c->ev1, c->stream));
-        uint32_t hc[4] = {0, 0, 0, 0};
+        uint32_t hc[8];
         if ((rc = read_counters(c, hc))) return rc;
         c->stats.ms_extend += elapsed(c->ev_a, c->ev_b);
         c->stats.ms_shade += elapsed(c->ev_b, c->ev1);
-        const uint32_t next_n = hc[0], nrec = hc[1];
-        if (MODE == 1 && wave_nee && nrec > 0) {
-            uint32_t nsr = hc[2];
+        if (shadow_pending) { // the stream waited for it before this shade
+            c->stats.rays_shadow += hc[4 * pend_slot + 3];
+            c->stats.ms_shadow += elapsed(c->ev_s0, c->ev_s1);
+            shadow_pending = false;
+        }
+        const uint32_t *h = hc + 4 * slot;
+        const uint32_t next_n = h[0], nrec = h[1];
+        if (overlap && nrec > 0) {
+            const uint32_t nsr = h[2];
+            c->stats.rays_shadow += nsr;
+            HIPX(c, hipEventRecord(c->ev_shaded, c->stream));
+            HIPX(c, hipStreamWaitEvent(c->stream2, c->ev_shaded, 0));
+            HIPX(c, hipEventRecord(c->ev_s0, c->stream2));
+            if (nsr > 0 && (rc = launch_trace(c, sc, true, nsr, sr[0].o, sr[0].d, sr[0].lp, nee.contrib, cnt + 3, c->stream2)))
+                return rc;
+            hipLaunchKernelGGL(k_nee_finalize, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream2, nrec, fp.num_lights, nee,
+                               Lpath);
+            HIPX(c, hipGetLastError());
+            HIPX(c, hipEventRecord(c->ev_s1, c->stream2));
+            HIPX(c, hipEventRecord(c->ev_nee, c->stream2));
+            shadow_pending = true;
+            pend_slot = slot;
+        } else if (MODE == 1 && wave_nee && nrec > 0) {
+            uint32_t nsr = h[2];
             int q = 0;
             HIPX(c, hipEventRecord(c->ev_a, c->stream));
-            if (fused_shadow(c) && nsr > 0) {
-                // one launch: glass continuations stay in their lanes; segments counted in counters[3]
-                c->stats.rays_shadow += nsr;
-                if ((rc = launch_trace(c, sc, true, nsr, sr[0].o, sr[0].d, sr[0].lp, nee.contrib, counters + 3))) return rc;
-                uint32_t h2[4];
-                if ((rc = read_counters(c, h2))) return rc;
-                c->stats.rays_shadow += h2[3];
-                nsr = 0;
-            }
             for (int step = 0; step < 4 && nsr > 0; step++) {
                 c->stats.rays_shadow += nsr;
                 if ((rc = launch_trace(c, sc, true, nsr, sr[q].o, sr[q].d, sr[q].lp))) return rc;
-                HIPX(c, hipMemsetAsync(counters + 3, 0, 4, c->stream));
+                HIPX(c, hipMemsetAsync(cnt + 3, 0, 4, c->stream));
                 hipLaunchKernelGGL(k_shadow_resolve, dim3(blocks_for(nsr)), dim3(kBlock), 0, c->stream, sc, nsr, sr[q],
-                                   static_cast<const uint4 *>(c->hitA.p), sr[q ^ 1], nee.contrib, counters + 3);
+                                   static_cast<const uint4 *>(c->hitA.p), sr[q ^ 1], nee.contrib, cnt + 3);
                 HIPX(c, hipGetLastError());
-                uint32_t h2[4];
+                uint32_t h2[8];
                 if ((rc = read_counters(c, h2))) return rc;
-                nsr = h2[3];
+                nsr = h2[4 * slot + 3];
                 q ^= 1;
             }
             hipLaunchKernelGGL(k_nee_finalize, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, nrec, fp.num_lights, nee,
@@ -788,19 +824,27 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
             HIPX(c, hipEventRecord(c->ev_a, c->stream));
             if (c->ref_order)
                 hipLaunchKernelGGL(k_shadow<true>, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, sc, fp, nrec, sh, Lpath,
-                                   counters);
+                                   cnt);
             else
                 hipLaunchKernelGGL(k_shadow<false>, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream, sc, fp, nrec, sh, Lpath,
-                                   counters);
+                                   cnt);
             HIPX(c, hipGetLastError());
             HIPX(c, hipEventRecord(c->ev_b, c->stream));
-            uint32_t h2[4];
+            uint32_t h2[8];
             if ((rc = read_counters(c, h2))) return rc;
             c->stats.ms_shadow += elapsed(c->ev_a, c->ev_b);
-            c->stats.rays_shadow += h2[2];
+            c->stats.rays_shadow += h2[4 * slot + 2];
         }
         n = next_n;
         cur ^= 1;
+    }
+    if (shadow_pending) {
+        HIPX(c, hipStreamWaitEvent(c->stream, c->ev_nee, 0));
+        uint32_t hc[8];
+        int rc = read_counters(c, hc);
+        if (rc) return rc;
+        c->stats.rays_shadow += hc[4 * pend_slot + 3];
+        c->stats.ms_shadow += elapsed(c->ev_s0, c->ev_s1);
     }
     HIPX(c, hipEventRecord(c->ev_a, c->stream));
     hipLaunchKernelGGL(k_resolve<MODE>, dim3(blocks_for(P)), dim3(kBlock), 0, c->stream, fp, s0, ns, Lpath,
