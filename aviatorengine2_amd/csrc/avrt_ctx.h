@@ -120,7 +120,7 @@ struct avrt_ctx {
     size_t cap_shadow = 0;
     int cu_count = 0;
     int trace_blocks_per_cu = 0;
-    int spec_blocks_per_cu = 0;
+    int ww_blocks_per_cu = 0;
 
     avrt::SceneRebuild rb;
     avrt::HybridState hy;

@@ -354,15 +354,18 @@ __device__ __forceinline__ bool test_box(const ObjGeom &g, F3 o, F3 d, float nea
 // child first), so hits at exactly equal t resolve as there; false visits the
 // nearer child first.
 //
-// node_visit() / leaf_phase() split the same walk into the two phases of a
-// speculative while-while traversal (Aila and Laine, "Understanding the
-// Efficiency of Ray Traversal on GPUs", HPG 2009): a lane that reaches a leaf
-// postpones it in `leaf` and keeps visiting interior nodes of the same tree
-// until it reaches a second leaf, so a wave runs its node visits and its leaf
-// tests as separate, fuller passes. Leaves are tested in the order step()
-// tests them; interior nodes visited while a leaf waits are culled against
-// the `far` from before that leaf, which can add visits but does not change
-// the closest hit.
+// node_visit() / leaf_phase() split the same walk into two kinds of step so
+// a persistent kernel can run a wave's node visits and its leaf tests as
+// separate passes, each with the lanes that need it (the while-while
+// structure of Aila and Laine, "Understanding the Efficiency of Ray Traversal
+// on GPUs", HPG 2009, without its speculation): a lane that reaches a leaf
+// holds it in `leaf` and visits no node until the leaf is tested. The lane
+// performs exactly the operations of step(), in the same order and with the
+// same `far` at every box test, so the hit is the same whatever the passes'
+// order. (Speculating past a held leaf would test boxes against a `far` the
+// leaf has not yet shortened; since the float entry distance of a box can
+// exceed by an ulp the float distance of a hit on its face, that can change
+// which of two hits at exactly equal distance wins.)
 //
 // FILTER restricts the objects considered:
 //   kAll        every object, closest hit;
@@ -381,7 +384,7 @@ constexpr int kVisitNext = 0;  // leaf done, continue with the next entry
 constexpr int kVisitEnter = 1; // entered an instance's BLAS, ref is its root
 constexpr int kVisitStop = 2;  // any-hit walk found its hit
 
-template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll, int LQ = 1>
+template <bool REF_ORDER, int K, int BLOCK, int SPILL, int FILTER = kAll>
 struct Traversal {
     F3 o, d, winv;    // world ray
     F3 ao, ainv;      // active ray (world, or mesh-local inside a BLAS)
@@ -389,8 +392,7 @@ struct Traversal {
     float time, near, far;
     float term; // kOpaqueAny: a hit closer than this ends the walk
     uint32_t ref;  // next entry to visit; kRefInvalid once the stack ran empty
-    uint32_t leaf;  // oldest postponed leaf of the speculative walk, kRefInvalid if none
-    uint32_t leaf2; // the next one (LQ = 2)
+    uint32_t leaf; // leaf reached and not yet tested (node_visit/leaf_phase), else kRefInvalid
     uint32_t cur_obj, tri_base;
     bool in_blas;
     uint32_t steps;
@@ -421,7 +423,6 @@ struct Traversal {
         tri_base = 0;
         ref = 0; // TLAS super-root
         leaf = kRefInvalid;
-        leaf2 = kRefInvalid;
         steps = 0;
         return sc.tlas != nullptr;
     }
@@ -456,44 +457,35 @@ struct Traversal {
             return next();
         }
         if ((ref & kLeafFlag) == 0) return interior(sc);
-        const int r = leaf_visit(sc, ref, kRefInvalid, kRefInvalid);
+        const int r = leaf_visit(sc, ref, kRefInvalid);
         if (r == kVisitEnter) return true;
         if (r == kVisitStop) return false;
         return next();
     }
 
-    // ---- speculative while-while walk (nearest-first) ----
-    // Moves leaves in ref into the free postponed slots (oldest in `leaf`),
-    // popping the next entry after each.
-    __device__ __forceinline__ static bool is_leaf(uint32_t r)
+    // ---- the same walk in node and leaf steps ----
+    // A leaf in ref becomes the held leaf and the next entry is popped (the
+    // order of the stack is unchanged: the popped entry goes back beneath an
+    // instance's BLAS if the leaf enters one).
+    __device__ __forceinline__ void hold_leaf()
     {
-        return (r & kLeafFlag) != 0 && r != kRefReturn && r != kRefInvalid;
-    }
-    __device__ __forceinline__ void postpone()
-    {
-        if (leaf == kRefInvalid && is_leaf(ref)) {
+        if (leaf == kRefInvalid && (ref & kLeafFlag) != 0 && ref != kRefReturn && ref != kRefInvalid) {
             leaf = ref;
             next();
         }
-        if (LQ > 1 && leaf2 == kRefInvalid && leaf != kRefInvalid && is_leaf(ref)) {
-            leaf2 = ref;
-            next();
-        }
     }
-    // An interior node to visit, or a BLAS to leave with no leaf of it waiting.
+    // A node to visit (or a BLAS to leave) and no leaf waiting to be tested.
     __device__ __forceinline__ bool node_ready() const
     {
-        return (ref & kLeafFlag) == 0 || (ref == kRefReturn && leaf == kRefInvalid);
+        return leaf == kRefInvalid && ((ref & kLeafFlag) == 0 || ref == kRefReturn);
     }
     __device__ __forceinline__ bool finished() const { return ref == kRefInvalid && leaf == kRefInvalid; }
 
-    // One visit of the node phase; only when node_ready().
+    // One node step; only when node_ready().
     __device__ __forceinline__ void node_visit(const SceneDev &sc)
     {
         if (++steps >= kMaxTraversalSteps) {
             ref = kRefInvalid;
-            leaf = kRefInvalid;
-            leaf2 = kRefInvalid;
             return;
         }
         if (ref == kRefReturn) {
@@ -502,30 +494,23 @@ struct Traversal {
         } else {
             interior(sc);
         }
-        postpone();
+        hold_leaf();
     }
 
-    // Tests the oldest postponed leaf; only when leaf != kRefInvalid.
-    // Entering an instance saves the speculative ref and the other postponed
-    // leaf on the stack beneath the rest of the leaf, so the walk resumes with
-    // them after the BLAS.
+    // Tests the held leaf; only when leaf != kRefInvalid. Entering an
+    // instance puts the popped next entry back on the stack beneath the rest
+    // of the leaf, where step() would have left it.
     __device__ __forceinline__ void leaf_phase(const SceneDev &sc)
     {
         const uint32_t lref = leaf;
-        leaf = leaf2;
-        leaf2 = kRefInvalid;
+        leaf = kRefInvalid;
         ++steps;
-        const int r = leaf_visit(sc, lref, ref, leaf);
+        const int r = leaf_visit(sc, lref, ref);
         if (r == kVisitStop) {
             ref = kRefInvalid;
-            leaf = kRefInvalid;
             return;
         }
-        if (r == kVisitEnter) {
-            leaf = kRefInvalid;
-            return;
-        }
-        postpone();
+        if (r == kVisitNext) hold_leaf();
     }
 
     // Interior node `ref`: tests the four children and moves ref to the next
@@ -604,9 +589,9 @@ struct Traversal {
     }
 
     // Leaf `lref`: tests its triangles (in a BLAS) or objects. Entering an
-    // instance pushes `pending` then `pending2` (each when valid), the rest of
-    // the leaf and the return marker, and sets ref to the BLAS root.
-    __device__ __forceinline__ int leaf_visit(const SceneDev &sc, uint32_t lref, uint32_t pending, uint32_t pending2)
+    // instance pushes `pending` (when valid), the rest of the leaf and the
+    // return marker, and sets ref to the BLAS root.
+    __device__ __forceinline__ int leaf_visit(const SceneDev &sc, uint32_t lref, uint32_t pending)
     {
         const uint32_t first = lref & 0x00FFFFFFu;
         const uint32_t count = (lref >> 24) & 0x7Fu;
@@ -660,7 +645,6 @@ struct Traversal {
                 // Leave the rest of this leaf for later and walk the
                 // instance's BLAS in mesh-local space (ToLocal).
                 if (pending != kRefInvalid) st.push(pending);
-                if (pending2 != kRefInvalid) st.push(pending2);
                 if (i + 1 < count) st.push(make_leaf(slot + 1, count - i - 1));
                 st.push(kRefReturn);
                 const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];

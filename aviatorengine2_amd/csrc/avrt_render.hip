@@ -55,8 +55,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev sc, FrameParams fp, u
                                                   const float *__restrict__ hitB, PathBufs out, ShadowBufs sh,
                                                   ShadowRays sr, NeeBufs nee, float4 *Lpath, uint32_t *counters)
 {
+    __shared__ float lut[256];
     shade_path<MODE, WAVE_NEE>(sc, fp, blockIdx.x * blockDim.x + threadIdx.x, n, bounce, in, hitA, hitB, out, sh, sr,
-                               nee, Lpath, counters);
+                               nee, Lpath, counters, srgb_to_lds(lut));
 }
 
 __global__ __launch_bounds__(kBlock) void k_shadow_resolve(SceneDev sc, uint32_t n, ShadowRays in,
@@ -352,8 +353,8 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
     c->trace_blocks_per_cu = nb;
     nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_spec<false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
-    c->spec_blocks_per_cu = nb;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_ww<false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
+    c->ww_blocks_per_cu = nb;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess || hybrid_init(c, lut) != AVRT_OK ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
@@ -655,7 +656,7 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
 }
 
 // True when the shadow-ray launch resolves glass continuations and occlusion
-// itself (the speculative persistent walk).
+// itself (the persistent node/leaf pass walk).
 static bool fused_shadow(const avrt_ctx *c) { return c->opts.trace_schedule == 0 && !c->ref_order; }
 
 // Closest-hit traversal of a ray queue into hitA/hitB; with fused_shadow(),
@@ -687,12 +688,16 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
         if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
         else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
     } else if (c->opts.trace_schedule == 0) {
-        const dim3 sgrid(std::max(1u, std::min(blocks_for(n), static_cast<uint32_t>(c->cu_count * c->spec_blocks_per_cu))));
+        const dim3 sgrid(std::max(1u, std::min(blocks_for(n), static_cast<uint32_t>(c->cu_count * c->ww_blocks_per_cu))));
+        // rays claimed per atomic: about 16 chunks per resident wave, a multiple of 64 in [64, 1024]
+        const uint32_t waves = sgrid.x * (kBlock / 64);
+        const uint32_t chunk = std::min(1024u, std::max(64u, (n / (16u * waves)) & ~63u));
         if (shadow) {
             if (contrib == nullptr || segs == nullptr) return set_err(c, AVRT_E_STATE, "shadow launch without contribution slots");
-            hipLaunchKernelGGL((k_trace_spec<true>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs);
+            hipLaunchKernelGGL((k_trace_ww<true>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk, contrib,
+                               segs);
         } else {
-            hipLaunchKernelGGL((k_trace_spec<false>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work,
+            hipLaunchKernelGGL((k_trace_ww<false>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk,
                                static_cast<float4 *>(nullptr), static_cast<uint32_t *>(nullptr));
         }
     } else {

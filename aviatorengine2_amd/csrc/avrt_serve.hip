@@ -134,12 +134,14 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
     const uint32_t b = jobs[j].path_base;
     const ShadowBufs none{nullptr, nullptr, nullptr, nullptr};
     const NeeBufs nj{nee.tp + b, nee.contrib + 4u * b};
+    __shared__ float lut_mem[256];
+    const float *lut = srgb_to_lds(lut_mem);
     if (jobs[j].mode == AVRT_MODE_PATH_TRACE)
         shade_path<0, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * j);
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * j, lut);
     else
         shade_path<1, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * j);
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * j, lut);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, const JobDev *__restrict__ jobs,

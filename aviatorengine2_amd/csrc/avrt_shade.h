@@ -28,6 +28,15 @@ constexpr uint32_t kWaveLights = 4;   // NEE as shadow-ray wavefronts up to this
 // sRGB-encoded byte -> linear, uploaded per translation unit.
 __constant__ float c_srgb[256];
 
+// Copies c_srgb into the workgroup's LDS table; every thread of the
+// workgroup calls it before any lookup.
+__device__ __forceinline__ const float *srgb_to_lds(float *table)
+{
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) table[i] = c_srgb[i];
+    __syncthreads();
+    return table;
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level queue compaction: returns this lane's output slot (valid only if
 // pred) with one atomic add per wave.
@@ -60,7 +69,10 @@ __device__ __forceinline__ int wrapi(int i, int n)
     return m < 0 ? m + n : m;
 }
 
-__device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t layer, float u, float v)
+// `lut` is the sRGB decode table: a copy in LDS where the kernel keeps one
+// (each texel costs three lookups), else c_srgb.
+__device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t layer, float u, float v,
+                            const float *lut = c_srgb)
 {
     if (fp.has_texture == 0) return f3(1.0f, 1.0f, 1.0f);
     if (layer >= fp.tex_layers) layer = fp.tex_layers - 1;
@@ -79,10 +91,10 @@ __device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t 
     const uchar4 b = base[static_cast<size_t>(j0) * W + i1];
     const uchar4 c = base[static_cast<size_t>(j1) * W + i0];
     const uchar4 e = base[static_cast<size_t>(j1) * W + i1];
-    const F3 t00 = f3(c_srgb[a.x], c_srgb[a.y], c_srgb[a.z]);
-    const F3 t10 = f3(c_srgb[b.x], c_srgb[b.y], c_srgb[b.z]);
-    const F3 t01 = f3(c_srgb[c.x], c_srgb[c.y], c_srgb[c.z]);
-    const F3 t11 = f3(c_srgb[e.x], c_srgb[e.y], c_srgb[e.z]);
+    const F3 t00 = f3(lut[a.x], lut[a.y], lut[a.z]);
+    const F3 t10 = f3(lut[b.x], lut[b.y], lut[b.z]);
+    const F3 t01 = f3(lut[c.x], lut[c.y], lut[c.z]);
+    const F3 t11 = f3(lut[e.x], lut[e.y], lut[e.z]);
     const F3 top = t00 + fx * (t10 - t00);
     const F3 bot = t01 + fx * (t11 - t01);
     return top + fy * (bot - top);
@@ -415,15 +427,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
     }
 }
 
-// Closest-hit traversal (nearest child first) by persistent lanes with the
-// speculative while-while walk of Traversal, scheduled per wave step: the
-// wave runs the pass that has the most lanes ready, either one interior node
-// per lane that can advance (node_ready) or one postponed leaf per lane that
-// holds one, BLAS and TLAS leaves in separate passes (their code differs).
+// Closest-hit traversal (nearest child first) by persistent lanes, the walk
+// split into node and leaf steps (Traversal::node_visit / leaf_phase) and
+// scheduled per wave step: the wave runs the pass that has the most lanes
+// ready, either one node step per lane that has a node to visit or one leaf
+// test per lane that holds a leaf, BLAS and TLAS leaves in separate passes
+// (their code differs). Each lane performs the walk of Traversal::step(), so
+// the hits equal those of the other schedules bit for bit.
 // Lanes whose ray is finished write its hit and, once kRefillMin of them are
-// free (or nothing else can run), claim new rays (one atomic per wave).
-// Every pass advances at least one lane, and every ray's walk is finite, so
-// the loop ends once the queue is drained and every lane is done.
+// free (or nothing else can run), take new rays from the wave's chunk of the
+// queue: `chunk` consecutive rays claimed with one atomic (all waves claim
+// from one counter, and same-address atomics serialise across the chip, so
+// one per refill would queue the waves behind each other). Every pass
+// advances at least one lane, and every ray's walk is finite, so the loop
+// ends once the queue is drained and every lane is done.
 //
 // SHADOW rays (the queue of ShadowRays) are resolved in place, with the
 // operations of shadow_resolve_ray: a hit on an opaque object before the
@@ -434,15 +451,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
 constexpr int kRefillMin = 8;
 
 template <bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace_spec(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+__global__ __launch_bounds__(kBlock) void k_trace_ww(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
                                                        const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
                                                        uint4 *__restrict__ hitA, float *__restrict__ hitB,
-                                                       uint32_t *work, float4 *__restrict__ contrib, uint32_t *segs)
+                                                       uint32_t *work, uint32_t chunk, float4 *__restrict__ contrib,
+                                                       uint32_t *segs)
 {
     __shared__ uint32_t lds_stack[kStackLds * kBlock];
     uint32_t spill_mem[kStackSpill];
     Traversal<false, kStackLds, kBlock, kStackSpill> tr;
     uint32_t idx = 0;
+    uint32_t cbase = 0, cend = 0; // the wave's unassigned rays [cbase, cend) (wave-uniform)
+    bool drained = false;         // the counter has passed n: claim no more
     uint32_t slot = 0, step = 0, nseg = 0; // SHADOW: contribution slot, glass steps taken, continuations
     bool have = false, done = false;
     while (true) {
@@ -488,12 +508,26 @@ __global__ __launch_bounds__(kBlock) void k_trace_spec(SceneDev sc, uint32_t n, 
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m_need >> 32),
                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m_need), 0u));
             const int leader = __ffsll(static_cast<unsigned long long>(m_need)) - 1;
-            uint32_t base = 0;
-            if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(work, cnt);
-            base = __shfl(base, leader);
-            const uint32_t avail = base < n ? min(cnt, n - base) : 0u;
-            if (need && rank < avail) {
-                idx = base + rank;
+            const uint32_t rem = cend - cbase;
+            uint32_t nb = n; // start of a newly claimed chunk
+            if (rem < cnt && !drained) {
+                if (static_cast<int>(__lane_id()) == leader) nb = atomicAdd(work, chunk);
+                nb = __shfl(nb, leader);
+                drained = nb >= n || n - nb <= chunk;
+            }
+            // lanes take the rest of the old chunk, then the new one
+            const uint32_t cand = rank < rem ? cbase + rank : (nb < n ? nb + (rank - rem) : n);
+            const uint32_t new_end = nb < n ? nb + min(chunk, n - nb) : n;
+            if (rem < cnt) {
+                const uint32_t used = cnt - rem; // taken from the new chunk
+                cbase = nb < n ? min(nb + used, new_end) : n;
+                cend = nb < n ? new_end : n;
+            } else {
+                cbase += cnt;
+            }
+            const bool got = rank < rem || cand < new_end;
+            if (need && got) {
+                idx = cand;
                 {
                     const float4 O = ro[idx];
                     const float4 D = rd[idx];
@@ -511,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_spec(SceneDev sc, uint32_t n, 
                         hitB[idx] = 0.0f;
                     }
                 }
-            } else if (need && avail == 0) {
+            } else if (need && drained) {
                 done = true; // the queue is drained
             }
             continue;
@@ -638,36 +672,35 @@ template <int MODE, bool WAVE_NEE>
 __device__ __forceinline__ void shade_path(const SceneDev &sc, const FrameParams &fp, uint32_t k, uint32_t n,
                                            uint32_t bounce, PathBufs in, const uint4 *__restrict__ hitA,
                                            const float *__restrict__ hitB, PathBufs out, ShadowBufs sh, ShadowRays sr,
-                                           NeeBufs nee, float4 *Lpath, uint32_t *counters)
+                                           NeeBufs nee, float4 *Lpath, uint32_t *counters, const float *lut)
 {
-    F3 lpt[kWaveLights], ldir[kWaveLights], lc[kWaveLights];
-    bool lray[kWaveLights];
-    for (uint32_t i = 0; i < kWaveLights; i++) {
-        lray[i] = false;
-        lpt[i] = f3(0, 0, 0);
-        ldir[i] = f3(0, 0, 0);
-        lc[i] = f3(0, 0, 0);
-    }
-    bool cont = false;
-    bool want_shadow = false;
+    // Stage 1: the hit's surface, emission and sky, and whether it scatters
+    // (`live`) and takes NEE (`want_shadow`).
+    bool live = false, want_shadow = false, cont = false;
+    F3 d = f3(0, 0, 0), tp = f3(0, 0, 0), salb = f3(0, 0, 0);
     F3 no = f3(0, 0, 0), nd = f3(0, 0, 0), ntp = f3(0, 0, 0);
+    F3 tp_at_hit = f3(0, 0, 0);
     float time = 0.0f;
-    uint32_t pid = 0, nrng = 0;
+    uint32_t pid = 0, rng_at_nee = 0;
+    Rng rng{0u};
     Surf s;
     s.pos = f3(0, 0, 0);
     s.normal = f3(0, 0, 0);
-    F3 albedo = f3(0, 0, 0), tp_at_hit = f3(0, 0, 0);
-    uint32_t rng_at_nee = 0;
+    s.u = 0.0f;
+    s.v = 0.0f;
+    s.face = false;
+    AvrtObject ob;
     if (k < n) {
         const float4 O = in.o[k];
         const float4 D = in.d[k];
         const float4 T = in.t[k];
         const uint4 HA = hitA[k];
-        const F3 o = f3(O.x, O.y, O.z), d = f3(D.x, D.y, D.z);
-        F3 tp = f3(T.x, T.y, T.z);
+        const F3 o = f3(O.x, O.y, O.z);
+        d = f3(D.x, D.y, D.z);
+        tp = f3(T.x, T.y, T.z);
         time = O.w;
         pid = __float_as_uint(D.w);
-        Rng rng{__float_as_uint(T.w)};
+        rng.s = __float_as_uint(T.w);
         HitRec h;
         h.t = __uint_as_float(HA.x);
         h.obj = HA.y;
@@ -682,104 +715,106 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const FrameParams
                 add_radiance(Lpath, pid, tp * ld3(fp.sky));
             }
         } else {
-            const AvrtObject ob = sc.objects[h.obj];
+            ob = sc.objects[h.obj];
             hit_attributes(sc, ob, h, o, d, time, s);
             const F3 alb = f3(ob.r, ob.g, ob.b);
             if (MODE == 0) {
                 F3 att = alb;
-                if (ob.textureID != AVRT_NO_TEXTURE) att = att * texture_color(sc, fp, ob.textureID, s.u, s.v);
+                if (ob.textureID != AVRT_NO_TEXTURE) att = att * texture_color(sc, fp, ob.textureID, s.u, s.v, lut);
                 if (ob.colorType == AVRT_DIFFUSE_LIGHT) add_radiance(Lpath, pid, tp * (att * ob.emission));
-                F3 b;
-                if (get_bounce(ob, d, s, rng, &b)) {
-                    no = s.pos;
-                    nd = b;
-                    ntp = tp * att;
-                    cont = (bounce + 1 < fp.depth);
-                }
+                salb = att;
+                live = true;
+            } else if (ob.colorType == AVRT_DIFFUSE_LIGHT) {
+                add_radiance(Lpath, pid, tp * (ob.emission * alb));
             } else {
-                if (ob.colorType == AVRT_DIFFUSE_LIGHT) {
-                    add_radiance(Lpath, pid, tp * (ob.emission * alb));
-                } else {
-                    F3 salb = alb;
-                    if (ob.textureID != AVRT_NO_TEXTURE) {
-                        const F3 texel = texture_color(sc, fp, ob.textureID, s.u, s.v);
-                        salb = texel * lerp(f3(1.0f, 1.0f, 1.0f), alb, ob.textureTint);
-                    }
-                    if (ob.colorType == AVRT_LAMBERTIAN && fp.num_lights > 0) {
-                        want_shadow = true;
-                        albedo = salb;
-                        tp_at_hit = tp;
-                        rng_at_nee = rng.s;
-                        if (WAVE_NEE) {
-                            // SampleLight and the early-outs of CalculateDirectLight
-                            // (:1130-1149); the occlusion test is deferred to k_trace.
-                            for (uint32_t i = 0; i < kWaveLights; i++) {
-                                if (i >= fp.num_lights) break;
-                                float dist;
-                                lray[i] = nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lpt[i], &ldir[i],
-                                                    &dist, &lc[i]);
-                            }
-                        } else {
-                            rng.s = rng.s * fp.jumpA + fp.jumpC;
-                        }
-                    }
-                    F3 sc_dir;
-                    if (get_bounce(ob, d, s, rng, &sc_dir)) {
-                        const F3 nd2 = normalize(sc_dir);
-                        const float sign = (dot(nd2, s.normal) < 0.0f) ? -1.0f : 1.0f;
-                        const F3 no2 = s.pos + s.normal * (0.001f * sign);
-                        if (ob.colorType != AVRT_DIELECTRIC) tp = tp * salb;
-                        bool alive = true;
-                        if (bounce > 0) {
-                            float p = fmaxf(tp.x, fmaxf(tp.y, tp.z));
-                            p = fminf(fmaxf(p, 0.1f), 1.0f);
-                            if (rng.next() > p) alive = false;
-                            else tp = f3(tp.x / p, tp.y / p, tp.z / p);
-                        }
-                        if (alive) {
-                            no = no2;
-                            nd = nd2;
-                            ntp = tp;
-                            cont = (bounce + 1 < fp.depth);
-                        }
-                    }
+                salb = alb;
+                if (ob.textureID != AVRT_NO_TEXTURE) {
+                    const F3 texel = texture_color(sc, fp, ob.textureID, s.u, s.v, lut);
+                    salb = texel * lerp(f3(1.0f, 1.0f, 1.0f), alb, ob.textureTint);
+                }
+                live = true;
+                if (ob.colorType == AVRT_LAMBERTIAN && fp.num_lights > 0) {
+                    want_shadow = true;
+                    tp_at_hit = tp;
+                    rng_at_nee = rng.s;
+                    if (!WAVE_NEE) rng.s = rng.s * fp.jumpA + fp.jumpC;
                 }
             }
         }
-        nrng = rng.s;
+    }
+    // Stage 2 (ray_trace, NEE as wavefronts): one record per Lambertian hit,
+    // then SampleLight and the early-outs of CalculateDirectLight for each
+    // light in order (:1130-1149), each unoccluded contribution stored and its
+    // shadow ray queued; the occlusion test is left to the shadow walk.
+    if (MODE == 1 && WAVE_NEE) {
+        const uint32_t rec = wave_append(want_shadow, &counters[1]);
+        const uint32_t nl = fp.num_lights;
+        if (want_shadow) nee.tp[rec] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, __uint_as_float(pid));
+        const F3 so = s.pos + (s.normal * 0.001f);
+        for (uint32_t i = 0; i < kWaveLights; i++) {
+            bool emit = false;
+            F3 lpt = f3(0, 0, 0), ldir = f3(0, 0, 0);
+            if (want_shadow && i < nl) {
+                F3 lc = f3(0, 0, 0);
+                float dist;
+                emit = nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lpt, &ldir, &dist, &lc);
+                nee.contrib[rec * nl + i] = make_float4(lc.x, lc.y, lc.z, 0.0f);
+            }
+            const uint32_t r = wave_append(emit, &counters[2]);
+            if (emit) {
+                sr.o[r] = make_float4(so.x, so.y, so.z, time);
+                sr.d[r] = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(rec * nl + i));
+                sr.lp[r] = make_float4(lpt.x, lpt.y, lpt.z, __uint_as_float(0u));
+            }
+        }
+    }
+    // Stage 3: the scattered ray (after NEE's random numbers).
+    if (live) {
+        if (MODE == 0) {
+            F3 b;
+            if (get_bounce(ob, d, s, rng, &b)) {
+                no = s.pos;
+                nd = b;
+                ntp = tp * salb;
+                cont = (bounce + 1 < fp.depth);
+            }
+        } else {
+            F3 sc_dir;
+            if (get_bounce(ob, d, s, rng, &sc_dir)) {
+                const F3 nd2 = normalize(sc_dir);
+                const float sign = (dot(nd2, s.normal) < 0.0f) ? -1.0f : 1.0f;
+                const F3 no2 = s.pos + s.normal * (0.001f * sign);
+                F3 t2 = tp;
+                if (ob.colorType != AVRT_DIELECTRIC) t2 = t2 * salb;
+                bool alive = true;
+                if (bounce > 0) {
+                    float p = fmaxf(t2.x, fmaxf(t2.y, t2.z));
+                    p = fminf(fmaxf(p, 0.1f), 1.0f);
+                    if (rng.next() > p) alive = false;
+                    else t2 = f3(t2.x / p, t2.y / p, t2.z / p);
+                }
+                if (alive) {
+                    no = no2;
+                    nd = nd2;
+                    ntp = t2;
+                    cont = (bounce + 1 < fp.depth);
+                }
+            }
+        }
     }
     const uint32_t slot = wave_append(cont, &counters[0]);
     if (cont) {
         out.o[slot] = make_float4(no.x, no.y, no.z, time);
         out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(pid));
-        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(nrng));
+        out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(rng.s));
     }
     if (MODE == 1 && !WAVE_NEE) {
         const uint32_t ss = wave_append(want_shadow, &counters[1]);
         if (want_shadow) {
             sh.p[ss] = make_float4(s.pos.x, s.pos.y, s.pos.z, time);
             sh.n[ss] = make_float4(s.normal.x, s.normal.y, s.normal.z, __uint_as_float(pid));
-            sh.a[ss] = make_float4(albedo.x, albedo.y, albedo.z, __uint_as_float(rng_at_nee));
+            sh.a[ss] = make_float4(salb.x, salb.y, salb.z, __uint_as_float(rng_at_nee));
             sh.w[ss] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, 0.0f);
-        }
-    }
-    if (MODE == 1 && WAVE_NEE) {
-        const uint32_t rec = wave_append(want_shadow, &counters[1]);
-        const uint32_t nl = fp.num_lights;
-        if (want_shadow) {
-            nee.tp[rec] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, __uint_as_float(pid));
-            for (uint32_t i = 0; i < kWaveLights; i++)
-                if (i < nl) nee.contrib[rec * nl + i] = make_float4(lc[i].x, lc[i].y, lc[i].z, 0.0f);
-        }
-        const F3 so = s.pos + (s.normal * 0.001f);
-        for (uint32_t i = 0; i < kWaveLights; i++) {
-            const bool emit = want_shadow && lray[i];
-            const uint32_t r = wave_append(emit, &counters[2]);
-            if (emit) {
-                sr.o[r] = make_float4(so.x, so.y, so.z, time);
-                sr.d[r] = make_float4(ldir[i].x, ldir[i].y, ldir[i].z, __uint_as_float(rec * nl + i));
-                sr.lp[r] = make_float4(lpt[i].x, lpt[i].y, lpt[i].z, __uint_as_float(0u));
-            }
         }
     }
 }

@@ -32,9 +32,9 @@ class Renderer:
     # -- lifecycle ---------------------------------------------------------
     def Initialize(self, device=0, max_width=0, max_height=0, max_paths=0, fp16_accum=False, reference_order=False,
                    lane_trace=False, trace_schedule=None):
-        """trace_schedule: 0 persistent lanes (speculative walk when nearest
-        first), 1 one ray per lane, 2 persistent lanes stepping one node or
-        leaf at a time; lane_trace=True is schedule 1."""
+        """trace_schedule: 0 persistent lanes (node and leaf passes when
+        nearest first), 1 one ray per lane, 2 persistent lanes stepping one
+        node or leaf at a time; lane_trace=True is schedule 1."""
         self.lib = abi.load_library()
         opts = AvrtCreateOptions()
         opts.device = device

@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # extension-ray traversal kernel of each trace schedule (nearest-first)
-TRACE_KERNEL = {0: "k_trace_spec<false>", 1: "k_trace_lane<false, false>", 2: "k_trace<false, false>"}
+TRACE_KERNEL = {0: "k_trace_ww<false>", 1: "k_trace_lane<false, false>", 2: "k_trace<false, false>"}
 
 
 def log(msg):
@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lane-trace", action="store_true", help="one ray per lane instead of persistent lanes")
     ap.add_argument("--trace-schedule", type=int, default=None,
-                    help="0 speculative persistent walk (default), 1 one ray per lane, 2 one-step persistent walk")
+                    help="0 persistent node/leaf passes (default), 1 one ray per lane, 2 one-step persistent walk")
     ap.add_argument("--served-jobs", type=int, default=160, help="jobs of the served stream (0 skips the leg)")
     ap.add_argument("--served-load", type=float, default=0.6,
                     help="Poisson arrival rate of the latency run, as a fraction of the measured served capacity")

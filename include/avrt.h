@@ -194,13 +194,13 @@ typedef struct AvrtCreateOptions {
                                (ray_trace.comp.hybrid.hlsl:909-913), so hits
                                at exactly equal t resolve as in the reference */
     uint32_t trace_schedule; /* 0: persistent lanes that take a new ray when
-                               theirs finishes; nearest-first walks run the
-                               speculative while-while traversal, each wave
-                               step running node visits or leaf tests for
-                               the lanes ready for them (fastest measured).
+                               theirs finishes; nearest-first walks run as
+                               node and leaf passes, each wave step running
+                               node visits or leaf tests for the lanes ready
+                               for them (fastest measured).
                                1: one ray per lane until it finishes.
                                2: persistent lanes, one node or leaf per
-                               step. Same hits.                              */
+                               step. Same hits, bit for bit.                 */
     uint32_t reserved[1];
 } AvrtCreateOptions;
 

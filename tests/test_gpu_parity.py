@@ -227,7 +227,7 @@ def test_chunked_paths_equal_single_wave(c2):
 
 @pytest.mark.parametrize("ref_order", [False, True])
 def test_trace_schedules_agree(c3, ref_order):
-    """The speculative persistent walk (default), one ray per lane and the
+    """The persistent node/leaf pass walk (default), one ray per lane and the
     one-step persistent walk give the same image, and the same primary hits."""
     say(f"traversal schedules 0/1/2, ref_order={ref_order}")
     cfg = small_cfg(c3.config, 160, 90, 2)
