@@ -41,6 +41,15 @@ using namespace avrt::dev;
 
 namespace {
 
+// Counter words: the claim counters of the two streams' trace launches at 0
+// and kCtr, then two bounce slots from kSlotBase; slot b & 1 of bounce b
+// holds {next queue size, NEE records, shadow rays, shadow continuation
+// segments}, kCtr words apart. read_counters copies both slots into host8
+// (slot s, counter i at host8[4 * s + i]).
+constexpr uint32_t kSlotBase = 2 * kCtr;
+constexpr uint32_t kSlotWords = 4 * kCtr;
+constexpr size_t kCounterBytes = (kSlotBase + 2 * kSlotWords) * sizeof(uint32_t);
+
 __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t first_sample, uint32_t ns, uint32_t n,
                                                    PathBufs out, float4 *Lpath)
 {
@@ -164,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev sc, FrameParams fp, 
         add_radiance(Lpath, pid, f3(W4.x, W4.y, W4.z) * sum);
     }
     const uint32_t tot = wave_sum(traced);
-    if (__lane_id() == 0 && tot) atomicAdd(&counters[2], tot);
+    if (__lane_id() == 0 && tot) atomicAdd(&counters[2 * kCtr], tot);
 }
 
 __global__ __launch_bounds__(kBlock) void k_display_only(FrameParams fp, const float4 *image, uchar4 *display)
@@ -595,7 +604,7 @@ static int ensure_paths(avrt_ctx *c, size_t n, uint32_t nlights)
         if ((rc = ensure(c, c->sha, n * 16))) return rc;
         if ((rc = ensure(c, c->shw, n * 16))) return rc;
         if ((rc = ensure(c, c->nee_tp, n * 16))) return rc;
-        if ((rc = ensure(c, c->counters, 256))) return rc;
+        if ((rc = ensure(c, c->counters, kCounterBytes))) return rc;
         c->cap_paths = n;
     }
     const size_t ns = n * (nlights <= kWaveLights ? nlights : 0);
@@ -681,7 +690,7 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
         HIPX(c, hipGetLastError());
         return AVRT_OK;
     }
-    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + (st == c->stream2 ? 9 : 8);
+    uint32_t *work = static_cast<uint32_t *>(c->counters.p) + (st == c->stream2 ? kCtr : 0);
     HIPX(c, hipMemsetAsync(work, 0, 4, st));
     const dim3 grid(trace_grid(c, n));
     if (c->ref_order) {
@@ -708,15 +717,14 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
     return AVRT_OK;
 }
 
-// Bounce counters: slot b & 1 of bounce b holds {next queue size, NEE records,
-// shadow rays, shadow continuation segments}; read_counters copies both slots.
-constexpr uint32_t kSlotBase = 16;
-
 static int read_counters(avrt_ctx *c, uint32_t *host8)
 {
-    HIPX(c, hipMemcpyAsync(host8, static_cast<uint32_t *>(c->counters.p) + kSlotBase, 32, hipMemcpyDeviceToHost,
+    uint32_t raw[2 * kSlotWords];
+    HIPX(c, hipMemcpyAsync(raw, static_cast<uint32_t *>(c->counters.p) + kSlotBase, sizeof(raw), hipMemcpyDeviceToHost,
                            c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
+    for (uint32_t s = 0; s < 2; s++)
+        for (uint32_t i = 0; i < 4; i++) host8[4 * s + i] = raw[s * kSlotWords + i * kCtr];
     return AVRT_OK;
 }
 
@@ -757,10 +765,10 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
     uint32_t pend_slot = 0;
     for (uint32_t b = 0; b < fp.depth && n > 0; b++) {
         const uint32_t slot = b & 1u;
-        uint32_t *cnt = counters + kSlotBase + 4 * slot;
+        uint32_t *cnt = counters + kSlotBase + kSlotWords * slot;
         if (b == 0) c->stats.rays_primary += n;
         else c->stats.rays_bounce += n;
-        HIPX(c, hipMemsetAsync(cnt, 0, 16, c->stream));
+        HIPX(c, hipMemsetAsync(cnt, 0, kSlotWords * sizeof(uint32_t), c->stream));
         HIPX(c, hipEventRecord(c->ev_a, c->stream));
         int rc = launch_trace(c, sc, false, n, buf[cur].o, buf[cur].d, nullptr);
         if (rc) return rc;
@@ -794,7 +802,8 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
             HIPX(c, hipEventRecord(c->ev_shaded, c->stream));
             HIPX(c, hipStreamWaitEvent(c->stream2, c->ev_shaded, 0));
             HIPX(c, hipEventRecord(c->ev_s0, c->stream2));
-            if (nsr > 0 && (rc = launch_trace(c, sc, true, nsr, sr[0].o, sr[0].d, sr[0].lp, nee.contrib, cnt + 3, c->stream2)))
+            if (nsr > 0 &&
+                (rc = launch_trace(c, sc, true, nsr, sr[0].o, sr[0].d, sr[0].lp, nee.contrib, cnt + 3 * kCtr, c->stream2)))
                 return rc;
             hipLaunchKernelGGL(k_nee_finalize, dim3(blocks_for(nrec)), dim3(kBlock), 0, c->stream2, nrec, fp.num_lights, nee,
                                Lpath);
@@ -810,9 +819,9 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
             for (int step = 0; step < 4 && nsr > 0; step++) {
                 c->stats.rays_shadow += nsr;
                 if ((rc = launch_trace(c, sc, true, nsr, sr[q].o, sr[q].d, sr[q].lp))) return rc;
-                HIPX(c, hipMemsetAsync(cnt + 3, 0, 4, c->stream));
+                HIPX(c, hipMemsetAsync(cnt + 3 * kCtr, 0, 4, c->stream));
                 hipLaunchKernelGGL(k_shadow_resolve, dim3(blocks_for(nsr)), dim3(kBlock), 0, c->stream, sc, nsr, sr[q],
-                                   static_cast<const uint4 *>(c->hitA.p), sr[q ^ 1], nee.contrib, cnt + 3);
+                                   static_cast<const uint4 *>(c->hitA.p), sr[q ^ 1], nee.contrib, cnt + 3 * kCtr);
                 HIPX(c, hipGetLastError());
                 uint32_t h2[8];
                 if ((rc = read_counters(c, h2))) return rc;

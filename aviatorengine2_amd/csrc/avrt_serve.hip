@@ -138,10 +138,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
     const float *lut = srgb_to_lds(lut_mem);
     if (jobs[j].mode == AVRT_MODE_PATH_TRACE)
         shade_path<0, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * j, lut);
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
     else
         shade_path<1, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * j, lut);
+                            at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
@@ -152,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, cons
     const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
     const uint32_t s = 4u * jobs[j].path_base;
     const SceneDev sc = jobs[j].sc;
-    shadow_resolve_ray(sc, local, L.count[j], at(in, s), hitA + s, at(out, s), contrib + s, counters + 4u * j + 3u);
+    shadow_resolve_ray(sc, local, L.count[j], at(in, s), hitA + s, at(out, s), contrib + s,
+                       counters + 4u * kCtr * j + 3u * kCtr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_nee_finalize(SegLaunch L, const JobDev *__restrict__ jobs, NeeBufs nee,
@@ -297,7 +298,7 @@ int ensure_round(avrt_server *s, size_t paths)
     }
     if ((rc = salloc(s, s->hitA, 4 * paths * 16)) || (rc = salloc(s, s->hitB, 4 * paths * 4)) ||
         (rc = salloc(s, s->Lpath, paths * 16)) || (rc = salloc(s, s->nee_tp, paths * 16)) ||
-        (rc = salloc(s, s->nee_contrib, 4 * paths * 16)) || (rc = salloc(s, s->counters, kMaxRoundJobs * 16)) ||
+        (rc = salloc(s, s->nee_contrib, 4 * paths * 16)) || (rc = salloc(s, s->counters, kMaxRoundJobs * 16 * kCtr)) ||
         (rc = salloc(s, s->jobtab, kMaxRoundJobs * sizeof(JobDev))))
         return rc;
     s->round_cap = paths;
@@ -318,10 +319,15 @@ uint32_t make_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
     return b;
 }
 
+// Job j's four counters are kCtr words apart from word 4 * kCtr * j (one
+// 128-byte line each); host[4 * j + i] receives counter i of job j.
 int read_job_counters(avrt_server *s, uint32_t n, uint32_t *host)
 {
-    SHIPX(s, hipMemcpyAsync(host, s->counters.p, 16 * n, hipMemcpyDeviceToHost, s->stream));
+    std::vector<uint32_t> raw(static_cast<size_t>(4) * kCtr * n);
+    SHIPX(s, hipMemcpyAsync(raw.data(), s->counters.p, raw.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
     SHIPX(s, hipStreamSynchronize(s->stream));
+    for (uint32_t j = 0; j < n; j++)
+        for (uint32_t i = 0; i < 4; i++) host[4 * j + i] = raw[(4 * j + i) * kCtr];
     return AVRT_OK;
 }
 
@@ -519,7 +525,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
         hipLaunchKernelGGL((k_seg_trace<false>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[cur].o, buf[cur].d,
                            static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>());
         SHIPX(s, hipGetLastError());
-        SHIPX(s, hipMemsetAsync(ctr, 0, 16 * J, s->stream));
+        SHIPX(s, hipMemsetAsync(ctr, 0, 16 * kCtr * J, s->stream));
         hipLaunchKernelGGL(k_seg_shade, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, b, buf[cur],
                            s->hitA.as<const uint4>(), s->hitB.as<const float>(), buf[cur ^ 1], sr[0], nee, Lpath, ctr);
         SHIPX(s, hipGetLastError());
@@ -542,7 +548,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
                 hipLaunchKernelGGL((k_seg_trace<true>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q].o, sr[q].d,
                                    sr[q].lp, s->hitA.as<uint4>(), s->hitB.as<float>());
                 SHIPX(s, hipGetLastError());
-                SHIPX(s, hipMemsetAsync(ctr, 0, 16 * J, s->stream));
+                SHIPX(s, hipMemsetAsync(ctr, 0, 16 * kCtr * J, s->stream));
                 hipLaunchKernelGGL(k_seg_shadow_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q],
                                    s->hitA.as<const uint4>(), sr[q ^ 1], nee.contrib, ctr);
                 SHIPX(s, hipGetLastError());

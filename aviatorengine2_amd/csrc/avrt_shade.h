@@ -25,6 +25,12 @@ constexpr int kStackSpill = 96;
 constexpr int kStepsPerRefill = 24;   // traversal steps between lane refills
 constexpr uint32_t kWaveLights = 4;   // NEE as shadow-ray wavefronts up to this many lights
 
+// Queue counters of a frame's queue set (next extension rays, NEE records,
+// shadow rays, shadow continuations) sit kCtr words apart, one per 128-byte
+// line: atomics on one line serialise across the chip, and every wave of a
+// shading launch bumps three of them.
+constexpr uint32_t kCtr = 32;
+
 // sRGB-encoded byte -> linear, uploaded per translation unit.
 __constant__ float c_srgb[256];
 
@@ -747,7 +753,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const FrameParams
     // light in order (:1130-1149), each unoccluded contribution stored and its
     // shadow ray queued; the occlusion test is left to the shadow walk.
     if (MODE == 1 && WAVE_NEE) {
-        const uint32_t rec = wave_append(want_shadow, &counters[1]);
+        const uint32_t rec = wave_append(want_shadow, &counters[kCtr]);
         const uint32_t nl = fp.num_lights;
         if (want_shadow) nee.tp[rec] = make_float4(tp_at_hit.x, tp_at_hit.y, tp_at_hit.z, __uint_as_float(pid));
         const F3 so = s.pos + (s.normal * 0.001f);
@@ -760,7 +766,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const FrameParams
                 emit = nee_light(sc, sc.lights[i], s.pos, s.normal, salb, rng, &lpt, &ldir, &dist, &lc);
                 nee.contrib[rec * nl + i] = make_float4(lc.x, lc.y, lc.z, 0.0f);
             }
-            const uint32_t r = wave_append(emit, &counters[2]);
+            const uint32_t r = wave_append(emit, &counters[2 * kCtr]);
             if (emit) {
                 sr.o[r] = make_float4(so.x, so.y, so.z, time);
                 sr.d[r] = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(rec * nl + i));
@@ -809,7 +815,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &sc, const FrameParams
         out.t[slot] = make_float4(ntp.x, ntp.y, ntp.z, __uint_as_float(rng.s));
     }
     if (MODE == 1 && !WAVE_NEE) {
-        const uint32_t ss = wave_append(want_shadow, &counters[1]);
+        const uint32_t ss = wave_append(want_shadow, &counters[kCtr]);
         if (want_shadow) {
             sh.p[ss] = make_float4(s.pos.x, s.pos.y, s.pos.z, time);
             sh.n[ss] = make_float4(s.normal.x, s.normal.y, s.normal.z, __uint_as_float(pid));
