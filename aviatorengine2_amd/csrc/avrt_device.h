@@ -179,6 +179,9 @@ __device__ __forceinline__ F3 rotate_y(F3 v, float s, float c) { return f3(c * v
 // pointer, so that only overflowing pushes and the pops that follow them touch
 // private memory; everything else in the traversal state stays in registers.
 // A push beyond K + SPILL entries sets `overflow` and drops the entry.
+// push_sorted() pushes up to three entries with three unconditional LDS
+// writes (an unused write lands on a free slot above the top) after one
+// check for ring headroom.
 template <int K, int BLOCK, int SPILL>
 struct ShortStack {
     uint32_t *col;   // this lane's column: lds + threadIdx.x
@@ -209,14 +212,44 @@ struct ShortStack {
         col[(sp & (K - 1)) * BLOCK] = v;
         sp++;
     }
+    // Pushes the first m of (r1, r2, r3) so that r1 ends on top (r_m
+    // deepest), as push(r_m) ... push(r1) would.
+    __device__ __forceinline__ void push_sorted(uint32_t r1, uint32_t r2, uint32_t r3, int m)
+    {
+        // keep 4 free ring slots: the 3 writes and the spare slot
+        while (sp - lo > K - 4) {
+            if (lo >= SPILL) {
+                overflow = true;
+                m = 0; // drop, like push() does
+                break;
+            }
+            spill[lo] = col[(lo & (K - 1)) * BLOCK];
+            lo++;
+        }
+        const int p1 = m >= 1 ? sp + m - 1 : sp + 3;
+        const int p2 = m >= 2 ? sp + m - 2 : sp + 3;
+        const int p3 = m >= 3 ? sp : sp + 3;
+        col[(p1 & (K - 1)) * BLOCK] = r1;
+        col[(p2 & (K - 1)) * BLOCK] = r2;
+        col[(p3 & (K - 1)) * BLOCK] = r3;
+        sp += m;
+    }
+    // The ring read is the common path; the spill read stays behind its own
+    // branch. The empty asm (no instruction, it only ties the register) keeps
+    // the compiler from folding the two reads into one generic-address load,
+    // which would send every pop through the vector memory pipeline.
     __device__ __forceinline__ uint32_t pop()
     {
         sp--;
-        if (sp < lo) {
+        uint32_t v = col[(sp & (K - 1)) * BLOCK];
+#if defined(__HIP_DEVICE_COMPILE__)
+        __asm__ volatile("" : "+v"(v));
+#endif
+        if (__builtin_expect(sp < lo, 0)) {
             lo = sp;
-            return spill[sp];
+            v = spill[sp];
         }
-        return col[(sp & (K - 1)) * BLOCK];
+        return v;
     }
 };
 // Slab test of one box (IntersectAABB): t0=(min-o)*inv, t1=(max-o)*inv,
@@ -580,9 +613,9 @@ struct Traversal {
             AVRT_CSWAP(k1, r1, k2, r2)
 #undef AVRT_CSWAP
             if (!(k0 < kMiss)) return next();
-            if (k3 < kMiss) st.push(r3);
-            if (k2 < kMiss) st.push(r2);
-            if (k1 < kMiss) st.push(r1);
+            // boxes that are hit have finite keys below kMiss, so after the
+            // sort they come first: push the hit ones among r1..r3
+            st.push_sorted(r1, r2, r3, (k1 < kMiss ? 1 : 0) + (k2 < kMiss ? 1 : 0) + (k3 < kMiss ? 1 : 0));
             ref = r0;
             return true;
         }
