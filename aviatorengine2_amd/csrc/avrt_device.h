@@ -597,14 +597,16 @@ struct Traversal {
             const float kMiss = 3.0e38f;
             float k0 = h0 ? t0 : kMiss, k1 = h1 ? t1 : kMiss, k2 = h2 ? t2 : kMiss, k3 = h3 ? t3 : kMiss;
             uint32_t r0 = R.x, r1 = R.y, r2 = R.z, r3 = R.w;
+// compare-exchange as selects (no branch, no exec-mask save/restore)
 #define AVRT_CSWAP(ka, ra, kb, rb)                                                                                  \
-    if (kb < ka) {                                                                                                 \
-        const float tk = ka;                                                                                       \
-        ka = kb;                                                                                                   \
-        kb = tk;                                                                                                   \
-        const uint32_t tr = ra;                                                                                    \
-        ra = rb;                                                                                                   \
-        rb = tr;                                                                                                   \
+    {                                                                                                              \
+        const bool sw = kb < ka;                                                                                   \
+        const float klo = sw ? kb : ka, khi = sw ? ka : kb;                                                        \
+        const uint32_t rlo = sw ? rb : ra, rhi = sw ? ra : rb;                                                     \
+        ka = klo;                                                                                                  \
+        kb = khi;                                                                                                  \
+        ra = rlo;                                                                                                  \
+        rb = rhi;                                                                                                  \
     }
             AVRT_CSWAP(k0, r0, k1, r1)
             AVRT_CSWAP(k2, r2, k3, r3)
