@@ -127,6 +127,19 @@ class AvrtServerStats(ctypes.Structure):
 
 JOB_QUEUED, JOB_ACTIVE, JOB_DONE = 0, 1, 2
 
+
+class AvrtTuning(ctypes.Structure):
+    _fields_ = [("stack_lds", ctypes.c_uint32), ("block", ctypes.c_uint32), ("sort_bits", ctypes.c_uint32),
+                ("samples_per_chunk", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class AvrtTuneReport(ctypes.Structure):
+    _fields_ = [("trials", ctypes.c_uint32), ("best", ctypes.c_uint32), ("trial", AvrtTuning * 32),
+                ("ms", ctypes.c_float * 32)]
+
 # Every entry point include/avrt.h declares: name -> (restype, argtypes).
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -147,6 +160,9 @@ _SIG = {
     "avrt_read_image": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "avrt_read_display": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "avrt_trace_primary": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "avrt_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(AvrtTuning)]),
+    "avrt_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(AvrtTuning)]),
+    "avrt_autotune": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(AvrtTuneReport)]),
     "avrt_build_bvh": (ctypes.c_int, [_P, _U32, _P, ctypes.POINTER(_U32), _P]),
     "avrt_build_blas": (ctypes.c_int, [_P, _U32, _P, _U32, _P, ctypes.POINTER(_U32), _P, _P]),
     "avrt_instance_bounds": (ctypes.c_int, [_P, _U32, _P, _P]),

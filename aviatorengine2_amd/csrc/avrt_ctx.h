@@ -71,6 +71,10 @@ struct HybridState {
 
 } // namespace avrt
 
+// Variants of the persistent pass walk (see kTraceVariants in avrt_render.hip).
+constexpr int kNumTraceVariants = 6;
+constexpr int kDefaultTraceVariant = 3; // 16-entry LDS stack, 256 threads
+
 struct avrt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -120,7 +124,17 @@ struct avrt_ctx {
     size_t cap_shadow = 0;
     int cu_count = 0;
     int trace_blocks_per_cu = 0;
-    int ww_blocks_per_cu = 0;
+    // tuning (avrt_set_tuning / avrt_autotune): pass-walk variant (LDS stack
+    // depth x block size), bounce-ray sort key bits (0: unsorted), sample
+    // frames per chunk (0: as many as max_paths allows)
+    int trace_variant = kDefaultTraceVariant;
+    uint32_t sort_bits = 0;
+    uint32_t chunk_spp = 0;
+    int ww_blocks[kNumTraceVariants] = {};
+    // bounce-ray sort: keys and values (two of each), the sorter's scratch, and
+    // a third set of ray buffers the sorted queue is gathered into
+    avrt::DevBuf rs_key[2], rs_val[2], rs_tmp, rs_o, rs_d, rs_t;
+    size_t cap_sort = 0, rs_tmp_bytes = 0;
 
     avrt::SceneRebuild rb;
     avrt::HybridState hy;

@@ -20,6 +20,7 @@
 // so every stage reads and writes it fully coalesced.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -49,6 +50,65 @@ namespace {
 constexpr uint32_t kSlotBase = 2 * kCtr;
 constexpr uint32_t kSlotWords = 4 * kCtr;
 constexpr size_t kCounterBytes = (kSlotBase + 2 * kSlotWords) * sizeof(uint32_t);
+
+// Variants of the persistent pass walk the autotuner picks from: LDS stack
+// ring entries per lane x workgroup size (the ray results are the same).
+constexpr uint32_t kTraceVariants[kNumTraceVariants][2] = {{8, 128}, {8, 256}, {16, 128}, {16, 256}, {32, 128}, {32, 256}};
+
+// Sort key of a bounce ray: its direction octant in the top 3 of `bits`
+// bits, then the leading bits of the Morton code (10 bits per axis) of its
+// origin in the scene box (the TLAS super-root's children).
+__global__ __launch_bounds__(kBlock) void k_sort_keys(SceneDev sc, uint32_t n, PathBufs in, uint32_t bits,
+                                                       uint32_t *keys, uint32_t *vals)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const Node4 &root = sc.tlas[0];
+    for (int c = 0; c < 4; c++) {
+        if (root.ref[c] == kRefInvalid) continue;
+        lo[0] = fminf(lo[0], root.lox[c]);
+        lo[1] = fminf(lo[1], root.loy[c]);
+        lo[2] = fminf(lo[2], root.loz[c]);
+        hi[0] = fmaxf(hi[0], root.hix[c]);
+        hi[1] = fmaxf(hi[1], root.hiy[c]);
+        hi[2] = fmaxf(hi[2], root.hiz[c]);
+    }
+    const float4 O = in.o[i], D = in.d[i];
+    const float p[3] = {O.x, O.y, O.z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        const float ext = hi[a] - lo[a];
+        const float f = ext > 0.0f ? (p[a] - lo[a]) / ext : 0.0f;
+        q[a] = static_cast<uint32_t>(fminf(fmaxf(f, 0.0f), 0.999999f) * 1024.0f);
+    }
+    uint32_t m = 0;
+    for (int b = 9; b >= 0; b--) m = (m << 3) | (((q[0] >> b) & 1u) << 2) | (((q[1] >> b) & 1u) << 1) | ((q[2] >> b) & 1u);
+    const uint32_t oct = (D.x < 0.0f ? 1u : 0u) | (D.y < 0.0f ? 2u : 0u) | (D.z < 0.0f ? 4u : 0u);
+    const uint32_t mb = bits - 3u; // Morton bits kept
+    keys[i] = (oct << mb) | (mb ? (m >> (30u - mb)) : 0u);
+    vals[i] = i;
+}
+
+// Resident workgroups per CU of pass-walk variant (K, B).
+template <int K, int B>
+static int ww_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_ww<false, K, B>, B, 0) != hipSuccess || nb <= 0) nb = 1;
+    return nb;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_rays(uint32_t n, const uint32_t *__restrict__ perm, PathBufs in,
+                                                         PathBufs out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    out.o[i] = in.o[j];
+    out.d[i] = in.d[j];
+    out.t[i] = in.t[j];
+}
 
 __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, uint32_t first_sample, uint32_t ns, uint32_t n,
                                                    PathBufs out, float4 *Lpath)
@@ -361,9 +421,12 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
     c->trace_blocks_per_cu = nb;
-    nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_ww<false>, kBlock, 0) != hipSuccess || nb <= 0) nb = 4;
-    c->ww_blocks_per_cu = nb;
+    c->ww_blocks[0] = ww_occupancy<8, 128>();
+    c->ww_blocks[1] = ww_occupancy<8, 256>();
+    c->ww_blocks[2] = ww_occupancy<16, 128>();
+    c->ww_blocks[3] = ww_occupancy<16, 256>();
+    c->ww_blocks[4] = ww_occupancy<32, 128>();
+    c->ww_blocks[5] = ww_occupancy<32, 256>();
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_srgb), lut, sizeof(lut)) != hipSuccess || hybrid_init(c, lut) != AVRT_OK ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
@@ -390,7 +453,8 @@ int avrt_destroy(avrt_ctx *c)
                       &c->sha, &c->shw, &c->counters, &c->rb.auth_objects, &c->rb.auth_inst, &c->rb.local,
                       &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
                       &c->rb.scratch, &c->rb.info, &c->rb.depth, &c->rb.range, &c->rb.lg_prefix, &c->rb.raster_prefix, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
-                      &c->srl[1], &c->nee_tp, &c->nee_contrib};
+                      &c->srl[1], &c->nee_tp, &c->nee_contrib, &c->rs_key[0], &c->rs_key[1], &c->rs_val[0],
+                      &c->rs_val[1], &c->rs_tmp, &c->rs_o, &c->rs_d, &c->rs_t};
     for (DevBuf *b : bufs) release(*b);
     hybrid_release(c);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b, c->ev_s0, c->ev_s1, c->ev_shaded, c->ev_nee})
@@ -664,6 +728,36 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
     return std::max(1u, std::min(blocks_for(n), resident));
 }
 
+// One launch of pass-walk variant (K, B): a resident grid, claim chunks of
+// about 16 per wave (a multiple of 64 in [64, 1024]).
+template <bool SHADOW, int K, int B>
+static void launch_ww_kb(const avrt_ctx *c, int v, hipStream_t st, const SceneDev &sc, uint32_t n, const float4 *ro,
+                         const float4 *rd, const float4 *rlp, uint4 *hA, float *hB, uint32_t *work, float4 *contrib,
+                         uint32_t *segs)
+{
+    const uint32_t resident = static_cast<uint32_t>(c->cu_count * c->ww_blocks[v]);
+    const uint32_t blocks = std::max(1u, std::min((n + B - 1) / B, resident));
+    const uint32_t waves = blocks * (B / 64);
+    const uint32_t chunk = std::min(1024u, std::max(64u, (n / (16u * waves)) & ~63u));
+    hipLaunchKernelGGL((k_trace_ww<SHADOW, K, B>), dim3(blocks), dim3(B), 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk,
+                       contrib, segs);
+}
+
+template <bool SHADOW>
+static void launch_ww(const avrt_ctx *c, hipStream_t st, const SceneDev &sc, uint32_t n, const float4 *ro, const float4 *rd,
+                      const float4 *rlp, uint4 *hA, float *hB, uint32_t *work, float4 *contrib, uint32_t *segs)
+{
+    const int v = c->trace_variant;
+    switch (v) {
+    case 0: launch_ww_kb<SHADOW, 8, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    case 1: launch_ww_kb<SHADOW, 8, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    case 2: launch_ww_kb<SHADOW, 16, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    case 4: launch_ww_kb<SHADOW, 32, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    case 5: launch_ww_kb<SHADOW, 32, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    default: launch_ww_kb<SHADOW, 16, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    }
+}
+
 // True when the shadow-ray launch resolves glass continuations and occlusion
 // itself (the persistent node/leaf pass walk).
 static bool fused_shadow(const avrt_ctx *c) { return c->opts.trace_schedule == 0 && !c->ref_order; }
@@ -697,17 +791,11 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
         if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
         else hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
     } else if (c->opts.trace_schedule == 0) {
-        const dim3 sgrid(std::max(1u, std::min(blocks_for(n), static_cast<uint32_t>(c->cu_count * c->ww_blocks_per_cu))));
-        // rays claimed per atomic: about 16 chunks per resident wave, a multiple of 64 in [64, 1024]
-        const uint32_t waves = sgrid.x * (kBlock / 64);
-        const uint32_t chunk = std::min(1024u, std::max(64u, (n / (16u * waves)) & ~63u));
         if (shadow) {
             if (contrib == nullptr || segs == nullptr) return set_err(c, AVRT_E_STATE, "shadow launch without contribution slots");
-            hipLaunchKernelGGL((k_trace_ww<true>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk, contrib,
-                               segs);
+            launch_ww<true>(c, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs);
         } else {
-            hipLaunchKernelGGL((k_trace_ww<false>), sgrid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk,
-                               static_cast<float4 *>(nullptr), static_cast<uint32_t *>(nullptr));
+            launch_ww<false>(c, st, sc, n, ro, rd, rlp, hA, hB, work, nullptr, nullptr);
         }
     } else {
         if (shadow) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
@@ -725,6 +813,48 @@ static int read_counters(avrt_ctx *c, uint32_t *host8)
     HIPX(c, hipStreamSynchronize(c->stream));
     for (uint32_t s = 0; s < 2; s++)
         for (uint32_t i = 0; i < 4; i++) host8[4 * s + i] = raw[s * kSlotWords + i * kCtr];
+    return AVRT_OK;
+}
+
+// Bounce-ray sort (sort_bits >= 3): keys from each ray's direction octant
+// and origin Morton code, a radix sort of (key, queue index) over the key
+// bits, and the rays gathered in key order into the spare buffer set, which
+// then becomes queue `which`. Path ids travel with the rays, so the order of
+// the queue changes no result.
+static int sort_queue(avrt_ctx *c, const SceneDev &sc, PathBufs &q, int which, uint32_t n)
+{
+    if (c->sort_bits < 3 || n < 2 || sc.tlas == nullptr) return AVRT_OK;
+    int rc;
+    const size_t paths = std::max<size_t>(c->cap_paths, n);
+    if ((rc = ensure(c, c->rs_o, paths * 16)) || (rc = ensure(c, c->rs_d, paths * 16)) ||
+        (rc = ensure(c, c->rs_t, paths * 16)))
+        return rc;
+    if (n > c->cap_sort) {
+        for (int i = 0; i < 2; i++)
+            if ((rc = ensure(c, c->rs_key[i], static_cast<size_t>(n) * 4)) || (rc = ensure(c, c->rs_val[i], static_cast<size_t>(n) * 4)))
+                return rc;
+        size_t tmp = 0;
+        HIPX(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, static_cast<const uint32_t *>(nullptr),
+                                                   static_cast<uint32_t *>(nullptr), static_cast<const uint32_t *>(nullptr),
+                                                   static_cast<uint32_t *>(nullptr), static_cast<int>(n), 0, 32, c->stream));
+        if ((rc = ensure(c, c->rs_tmp, tmp))) return rc;
+        c->rs_tmp_bytes = tmp;
+        c->cap_sort = n;
+    }
+    uint32_t *k0 = c->rs_key[0].as<uint32_t>(), *k1 = c->rs_key[1].as<uint32_t>();
+    uint32_t *v0 = c->rs_val[0].as<uint32_t>(), *v1 = c->rs_val[1].as<uint32_t>();
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, sc, n, q, c->sort_bits, k0, v0);
+    HIPX(c, hipGetLastError());
+    size_t tmp = c->rs_tmp_bytes;
+    HIPX(c, hipcub::DeviceRadixSort::SortPairs(c->rs_tmp.p, tmp, k0, k1, v0, v1, static_cast<int>(n), 0,
+                                               static_cast<int>(c->sort_bits), c->stream));
+    const PathBufs sorted{c->rs_o.as<float4>(), c->rs_d.as<float4>(), c->rs_t.as<float4>()};
+    hipLaunchKernelGGL(k_gather_rays, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream, n, v1, q, sorted);
+    HIPX(c, hipGetLastError());
+    std::swap(c->so[which], c->rs_o);
+    std::swap(c->sd[which], c->rs_d);
+    std::swap(c->st[which], c->rs_t);
+    q = PathBufs{c->so[which].as<float4>(), c->sd[which].as<float4>(), c->st[which].as<float4>()};
     return AVRT_OK;
 }
 
@@ -851,6 +981,13 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
         }
         n = next_n;
         cur ^= 1;
+        if (c->sort_bits >= 3 && n > 1 && b + 1 < fp.depth) {
+            HIPX(c, hipEventRecord(c->ev_a, c->stream));
+            if ((rc = sort_queue(c, sc, buf[cur], cur, n))) return rc;
+            HIPX(c, hipEventRecord(c->ev_b, c->stream));
+            HIPX(c, hipEventSynchronize(c->ev_b));
+            c->stats.ms_other += elapsed(c->ev_a, c->ev_b);
+        }
     }
     if (shadow_pending) {
         HIPX(c, hipStreamWaitEvent(c->stream, c->ev_nee, 0));
@@ -916,7 +1053,8 @@ int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
     const uint32_t S = fp.spp;
     size_t cap = c->opts.max_paths ? c->opts.max_paths : static_cast<size_t>(P) * S;
     if (cap < P) cap = P;
-    const uint32_t per_chunk = static_cast<uint32_t>(std::min<size_t>(S, cap / P));
+    uint32_t per_chunk = static_cast<uint32_t>(std::min<size_t>(S, cap / P));
+    if (c->chunk_spp) per_chunk = std::min(per_chunk, c->chunk_spp);
     rc = ensure_paths(c, static_cast<size_t>(P) * per_chunk, c->nlights);
     if (rc) return rc;
     for (uint32_t s0 = 0; s0 < S; s0 += per_chunk) {
@@ -1000,6 +1138,92 @@ int avrt_trace_primary(avrt_ctx *c, const AvrtConfig *cfg, uint32_t *hit_object,
         if (hit_t) hit_t[p] = hit ? t : 0.0f;
     }
     return AVRT_OK;
+}
+
+int avrt_get_tuning(const avrt_ctx *c, AvrtTuning *out)
+{
+    if (c == nullptr || out == nullptr) return AVRT_E_INVALID_ARG;
+    out->stack_lds = kTraceVariants[c->trace_variant][0];
+    out->block = kTraceVariants[c->trace_variant][1];
+    out->sort_bits = c->sort_bits;
+    out->samples_per_chunk = c->chunk_spp;
+    return AVRT_OK;
+}
+
+int avrt_set_tuning(avrt_ctx *c, const AvrtTuning *t)
+{
+    if (c == nullptr || t == nullptr) return AVRT_E_INVALID_ARG;
+    int v = -1;
+    for (int i = 0; i < kNumTraceVariants; i++)
+        if (kTraceVariants[i][0] == t->stack_lds && kTraceVariants[i][1] == t->block) v = i;
+    if (v < 0) return set_err(c, AVRT_E_INVALID_ARG, "stack_lds must be 8, 16 or 32 and block 128 or 256");
+    if (t->sort_bits != 0 && (t->sort_bits < 3 || t->sort_bits > 32))
+        return set_err(c, AVRT_E_INVALID_ARG, "sort_bits must be 0 or 3..32");
+    c->trace_variant = v;
+    c->sort_bits = t->sort_bits;
+    c->chunk_spp = t->samples_per_chunk;
+    return AVRT_OK;
+}
+
+int avrt_autotune(avrt_ctx *c, const AvrtConfig *cfg, int mode, AvrtTuneReport *report)
+{
+    if (c == nullptr || cfg == nullptr) return AVRT_E_INVALID_ARG;
+    AvrtTuneReport rep{};
+    AvrtTuning cur{};
+    int rc = avrt_get_tuning(c, &cur);
+    if (rc) return rc;
+    // one warm-up and one timed frame per setting; the frame's device time
+    auto trial = [&](const AvrtTuning &t, float *ms) -> int {
+        int r = avrt_set_tuning(c, &t);
+        if (r) return r;
+        if ((r = avrt_render(c, cfg, mode))) return r;
+        if ((r = avrt_render(c, cfg, mode))) return r;
+        *ms = c->stats.ms_total;
+        if (rep.trials < 32) {
+            rep.trial[rep.trials] = t;
+            rep.ms[rep.trials] = *ms;
+            rep.trials++;
+        }
+        return AVRT_OK;
+    };
+    AvrtTuning best = cur;
+    float best_ms = INFINITY;
+    auto consider = [&](const AvrtTuning &t) -> int {
+        float ms = 0.0f;
+        const int r = trial(t, &ms);
+        if (r) return r;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+        return AVRT_OK;
+    };
+    for (int v = 0; v < kNumTraceVariants; v++) {
+        AvrtTuning t{kTraceVariants[v][0], kTraceVariants[v][1], 0u, 0u};
+        if ((rc = consider(t))) break;
+    }
+    for (uint32_t bits : {12u, 24u}) {
+        if (rc) break;
+        AvrtTuning t = best;
+        t.sort_bits = bits;
+        rc = consider(t);
+    }
+    const uint32_t spp = cfg->samples ? cfg->samples : 1u;
+    for (uint32_t div : {2u, 4u}) {
+        if (rc || spp < div) break;
+        AvrtTuning t = best;
+        t.samples_per_chunk = spp / div;
+        rc = consider(t);
+    }
+    if (rc) {
+        (void)avrt_set_tuning(c, &cur);
+        return rc;
+    }
+    rc = avrt_set_tuning(c, &best);
+    for (uint32_t i = 0; i < rep.trials; i++)
+        if (std::memcmp(&rep.trial[i], &best, sizeof(best)) == 0 && rep.ms[i] == best_ms) rep.best = i;
+    if (report) *report = rep;
+    return rc;
 }
 
 } // extern "C"

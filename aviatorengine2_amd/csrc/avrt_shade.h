@@ -456,16 +456,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev sc, uint32_t n, const
 // added to *segs (one atomic per wave).
 constexpr int kRefillMin = 8;
 
-template <bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace_ww(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+// K: LDS stack ring entries per lane; BLOCK: threads per workgroup (both
+// picked by the autotuner among the instantiated values).
+template <bool SHADOW, int K = kStackLds, int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_trace_ww(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
                                                        const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
                                                        uint4 *__restrict__ hitA, float *__restrict__ hitB,
                                                        uint32_t *work, uint32_t chunk, float4 *__restrict__ contrib,
                                                        uint32_t *segs)
 {
-    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    __shared__ uint32_t lds_stack[K * BLOCK];
     uint32_t spill_mem[kStackSpill];
-    Traversal<false, kStackLds, kBlock, kStackSpill> tr;
+    Traversal<false, K, BLOCK, kStackSpill> tr;
     uint32_t idx = 0;
     uint32_t cbase = 0, cend = 0; // the wave's unassigned rays [cbase, cend) (wave-uniform)
     bool drained = false;         // the counter has passed n: claim no more

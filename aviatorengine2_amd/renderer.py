@@ -151,6 +151,26 @@ class Renderer:
         self._check(self.lib.avrt_read_display(self.ctx, ptr(out), out.nbytes), "avrt_read_display")
         return out
 
+    # -- tuning (traversal variant, bounce-ray sort, chunking) -------------
+    def GetTuning(self) -> dict:
+        t = abi.AvrtTuning()
+        self._check(self.lib.avrt_get_tuning(self.ctx, ctypes.byref(t)), "avrt_get_tuning")
+        return t.as_dict()
+
+    def SetTuning(self, stack_lds=16, block=256, sort_bits=0, samples_per_chunk=0):
+        t = abi.AvrtTuning(stack_lds, block, sort_bits, samples_per_chunk)
+        self._check(self.lib.avrt_set_tuning(self.ctx, ctypes.byref(t)), "avrt_set_tuning")
+
+    def Autotune(self, config, mode) -> dict:
+        """Times the settings on this scene and config, keeps the fastest.
+        Returns {"best": setting, "trials": [(setting, ms), ...]}."""
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        rep = abi.AvrtTuneReport()
+        self._check(self.lib.avrt_autotune(self.ctx, c.ctypes.data_as(ctypes.c_void_p), int(mode), ctypes.byref(rep)),
+                    "avrt_autotune")
+        trials = [(rep.trial[i].as_dict(), float(rep.ms[i])) for i in range(rep.trials)]
+        return {"best": trials[rep.best][0] if trials else self.GetTuning(), "trials": trials}
+
     def TracePrimary(self, config):
         c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
         P = int(c["width"][0]) * int(c["height"][0])

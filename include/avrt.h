@@ -325,6 +325,37 @@ int avrt_read_lighting_reduced(avrt_ctx *ctx, float *dst, size_t cells);
 int avrt_trace_primary(avrt_ctx *ctx, const AvrtConfig *config,
                        uint32_t *hit_object, uint32_t *hit_prim, float *hit_t);
 
+/* Tuning of avrt_render with trace_schedule 0 and nearest-first order. No
+ * counterpart in the reference (its dispatch has no knobs). Every setting
+ * gives the same image and counters, bit for bit; only the time differs. */
+typedef struct AvrtTuning {
+    uint32_t stack_lds;         /* LDS stack ring entries per lane: 8, 16 or 32 */
+    uint32_t block;             /* traversal workgroup size: 128 or 256          */
+    uint32_t sort_bits;         /* bounce-ray sort key: 0 (queue order kept) or
+                                   3..32 bits: direction octant, then the
+                                   origin's Morton code in the scene box        */
+    uint32_t samples_per_chunk; /* sample frames traced together (the queue
+                                   capacity in camera frames); 0 = as many as
+                                   max_paths allows                              */
+} AvrtTuning;
+
+typedef struct AvrtTuneReport {
+    uint32_t trials;            /* settings timed                                */
+    uint32_t best;              /* index of the setting now in use               */
+    AvrtTuning trial[32];
+    float ms[32];               /* device ms of one frame with each setting      */
+} AvrtTuneReport;
+
+int avrt_get_tuning(const avrt_ctx *ctx, AvrtTuning *out);
+/* AVRT_E_INVALID_ARG for values outside the lists above. */
+int avrt_set_tuning(avrt_ctx *ctx, const AvrtTuning *tuning);
+/* Autotuner: renders `config` (one warm-up and one timed frame per setting)
+ * with each traversal variant, then the best with sort keys of 12 and 24
+ * bits, then with 1/2 and 1/4 of the sample frames per chunk, keeps the
+ * fastest and reports all trials. Needs an uploaded scene; overwrites the
+ * accumulation image (render again afterwards). */
+int avrt_autotune(avrt_ctx *ctx, const AvrtConfig *config, int mode, AvrtTuneReport *report);
+
 /* ------------------------------------------------------------------------- */
 /* Acceleration structures.                                                  */
 /* ------------------------------------------------------------------------- */

@@ -11,6 +11,7 @@ Bars:
   * accumulation over calls equals one call with all samples;
   * error paths return the documented codes.
 """
+import os
 import sys
 
 import numpy as np
@@ -196,6 +197,65 @@ def test_fp16_accumulation_matches_oracle(c2):
     ref, _, _, _ = oracle.render(c2, cfg, abi.MODE_RAY_TRACE, fp16=True)
     _compare("fp16", img, ref)
     assert np.array_equal(img[..., :3], img[..., :3].astype(np.float16).astype(np.float32))
+
+
+@pytest.mark.parametrize("which", ["sphere_field", "icosphere_tris"])
+def test_converged_image_within_one_percent(c1, c2, which):
+    """4096 spp: the GPU image against the oracle's converged image of the
+    same PCG streams, relative RMSE <= 1 % (the images' own Monte-Carlo noise
+    at 4096 spp is well below that) and PSNR >= 50 dB."""
+    pk, mode, depth = (c1, abi.MODE_PATH_TRACE, 10) if which == "sphere_field" else (c2, abi.MODE_RAY_TRACE, 5)
+    cfg = small_cfg(pk.config, 48, 27, 4096, depth=depth)
+    say(f"converged {which} 48x27x4096")
+    img, _, _ = render_gpu(pk, cfg, mode)
+    ref, _, secs, _ = oracle.render(pk, cfg, mode, threads=min(16, os.cpu_count() or 1))
+    rep = imagecmp.report(img, ref)
+    say(f"converged {which}: {rep} (oracle {secs:.1f} s)")
+    assert rep["rmse_rel"] <= 0.01, rep
+    assert rep["psnr_db"] >= 50.0, rep
+
+
+def _counts(st):
+    return st["rays_primary"], st["rays_bounce"], st["rays_shadow"]
+
+
+def test_tuning_settings_give_identical_images(c3):
+    """Every traversal variant (LDS stack depth x workgroup size), bounce-ray
+    sort width and chunking gives the default's image and ray counts, bit for
+    bit; out-of-range settings are refused."""
+    cfg = small_cfg(c3.config, 160, 90, 4)
+    with Renderer().Initialize() as r:
+        r.LoadScene(c3)
+        assert r.GetTuning() == {"stack_lds": 16, "block": 256, "sort_bits": 0, "samples_per_chunk": 0}
+        ref_counts = _counts(r.RenderFrame(cfg, abi.MODE_RAY_TRACE))
+        ref = r.ReadImage()
+        for k in (8, 16, 32):
+            for blk in (128, 256):
+                for bits, chunk in ((0, 0), (12, 2), (32, 1)):
+                    say(f"tuning stack {k} block {blk} sort {bits} chunk {chunk}")
+                    r.SetTuning(k, blk, bits, chunk)
+                    counts = _counts(r.RenderFrame(cfg, abi.MODE_RAY_TRACE))
+                    assert counts == ref_counts, (k, blk, bits, chunk)
+                    assert np.array_equal(r.ReadImage(), ref), (k, blk, bits, chunk)
+        for bad in ((12, 256, 0, 0), (16, 64, 0, 0), (16, 256, 2, 0), (16, 256, 33, 0)):
+            with pytest.raises(AvrtError):
+                r.SetTuning(*bad)
+
+
+def test_autotune_keeps_the_fastest(c3):
+    cfg = small_cfg(c3.config, 160, 90, 4)
+    with Renderer().Initialize() as r:
+        r.LoadScene(c3)
+        r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
+        ref = r.ReadImage()
+        rep = r.Autotune(cfg, abi.MODE_RAY_TRACE)
+        say(f"autotune: best {rep['best']}, {len(rep['trials'])} trials")
+        assert len(rep["trials"]) == 6 + 2 + 2
+        best_ms = min(ms for _, ms in rep["trials"])
+        assert (rep["best"], best_ms) in rep["trials"]
+        assert r.GetTuning() == rep["best"]
+        r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
+        assert np.array_equal(r.ReadImage(), ref)
 
 
 def test_accumulation_across_calls_equals_one_call(c1):
