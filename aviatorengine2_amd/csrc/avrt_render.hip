@@ -140,9 +140,9 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams fp, uint32_t first_sample, uint32_t ns,
                                                     const float4 *Lpath, float4 *image, uchar4 *display)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= fp.width * fp.height) return;
-    resolve_pixel<MODE>(fp, p, first_sample, ns, Lpath, image, display);
+    const uint32_t bq = blockIdx.x * blockDim.x + threadIdx.x; // band position of the pixel
+    if (bq >= fp.width * fp.height) return;
+    resolve_pixel<MODE>(fp, bq, first_sample, ns, Lpath, image, display);
 }
 
 template <bool REF_ORDER>

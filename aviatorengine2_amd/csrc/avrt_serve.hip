@@ -180,13 +180,13 @@ __global__ __launch_bounds__(kBlock) void k_seg_resolve(SegLaunch L, const JobDe
                                                         const float4 *Lpath)
 {
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t p = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
-    if (p >= L.count[j]) return;
+    const uint32_t bq = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x; // band position of the pixel
+    if (bq >= L.count[j]) return;
     const JobDev &J = jobs[j];
     if (J.mode == AVRT_MODE_PATH_TRACE)
-        resolve_pixel<0>(J.fp, p, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+        resolve_pixel<0>(J.fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
     else
-        resolve_pixel<1>(J.fp, p, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+        resolve_pixel<1>(J.fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
 }
 
 } // namespace

@@ -877,16 +877,19 @@ __device__ __forceinline__ void shadow_resolve_ray(const SceneDev &sc, uint32_t 
 }
 
 
-// Pixel p: the `ns` sample frames of the chunk folded into the accumulation in
-// frame order, then the display (ray_trace.comp.hybrid.hlsl:1380-1403;
-// path_trace.comp.hlsl:851-859). Lpath holds the chunk's radiance, sample
-// major.
+// The pixel at band position `bq` (band_pixel): the `ns` sample frames of the
+// chunk folded into the accumulation in frame order, then the display
+// (ray_trace.comp.hybrid.hlsl:1380-1403; path_trace.comp.hlsl:851-859).
+// Lpath holds the chunk's radiance pixel-major in band order, so threads
+// taking consecutive band positions read consecutive runs of it.
 template <int MODE>
-__device__ __forceinline__ void resolve_pixel(const FrameParams &fp, uint32_t p, uint32_t first_sample, uint32_t ns,
+__device__ __forceinline__ void resolve_pixel(const FrameParams &fp, uint32_t bq, uint32_t first_sample, uint32_t ns,
                                               const float4 *Lpath, float4 *image, uchar4 *display)
 {
-    const uint32_t x = p % fp.width, y = p / fp.width;
-    const size_t q = static_cast<size_t>(band_index(x, y, fp.width, fp.height)) * ns;
+    uint32_t x, y;
+    band_pixel(bq, fp.width, fp.height, &x, &y);
+    const uint32_t p = y * fp.width + x;
+    const size_t q = static_cast<size_t>(bq) * ns;
     float4 acc = image[p];
     for (uint32_t s = 0; s < ns; s++) {
         const uint32_t batch = fp.batch + first_sample + s;

@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # extension-ray traversal kernel of each trace schedule (nearest-first)
-TRACE_KERNEL = {0: "k_trace_ww<false>", 1: "k_trace_lane<false, false>", 2: "k_trace<false, false>"}
+TRACE_KERNEL = {0: "k_trace_ww<false, 16, 256>", 1: "k_trace_lane<false, false>", 2: "k_trace<false, false>"}
 
 
 def log(msg):
@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=8,
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run avrt_autotune before the timed steps (default: the default tuning, which the profiles describe)")
     ap.add_argument("--lane-trace", action="store_true", help="one ray per lane instead of persistent lanes")
     ap.add_argument("--trace-schedule", type=int, default=None,
                     help="0 persistent node/leaf passes (default), 1 one ray per lane, 2 one-step persistent walk")
@@ -115,6 +117,13 @@ def main():
         r.SceneStep(spec.velocities, dt)
         s = r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
         log(f"warmup {w}: {s['ms_total']:.1f} ms")
+    tuning = None
+    if args.autotune and sched == 0:
+        t0 = time.perf_counter()
+        tune = r.Autotune(cfg, abi.MODE_RAY_TRACE)
+        tuning = {"best": tune["best"], "trials": [dict(t, ms=round(ms, 3)) for t, ms in tune["trials"]],
+                  "seconds": time.perf_counter() - t0}
+        log(f"autotune ({tuning['seconds']:.1f}s): {tune['best']}")
 
     tot = {"rays_primary": 0, "rays_bounce": 0, "rays_shadow": 0, "paths": 0, "ms_extend": 0.0,
            "ms_shade": 0.0, "ms_shadow": 0.0, "ms_other": 0.0, "launches_extend": 0, "ms_total": 0.0}
@@ -182,6 +191,7 @@ def main():
                                                                 "ms_other")},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "served": served,
         "hybrid": hyb,
     }
