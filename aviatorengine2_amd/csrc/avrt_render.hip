@@ -766,7 +766,8 @@ static bool fused_shadow(const avrt_ctx *c) { return c->opts.trace_schedule == 0
 // a shadow queue is resolved in place instead (contrib, segs). Launched on
 // `st` (the context's stream when null); each stream has its own claim counter.
 static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n, const float4 *ro, const float4 *rd,
-                        const float4 *rlp, float4 *contrib = nullptr, uint32_t *segs = nullptr, hipStream_t st = nullptr)
+                        const float4 *rlp, float4 *contrib = nullptr, uint32_t *segs = nullptr, hipStream_t st = nullptr,
+                        bool work_cleared = false)
 {
     if (st == nullptr) st = c->stream;
     uint4 *hA = static_cast<uint4 *>(c->hitA.p);
@@ -785,7 +786,7 @@ static int launch_trace(avrt_ctx *c, const SceneDev &sc, bool shadow, uint32_t n
         return AVRT_OK;
     }
     uint32_t *work = static_cast<uint32_t *>(c->counters.p) + (st == c->stream2 ? kCtr : 0);
-    HIPX(c, hipMemsetAsync(work, 0, 4, st));
+    if (!work_cleared) HIPX(c, hipMemsetAsync(work, 0, 4, st));
     const dim3 grid(trace_grid(c, n));
     if (c->ref_order) {
         if (shadow) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sc, n, ro, rd, rlp, hA, hB, work);
@@ -899,8 +900,11 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
         if (b == 0) c->stats.rays_primary += n;
         else c->stats.rays_bounce += n;
         HIPX(c, hipMemsetAsync(cnt, 0, kSlotWords * sizeof(uint32_t), c->stream));
+        // the walk's claim counter is cleared outside the timed bracket, so the
+        // events time the traversal kernel alone
+        HIPX(c, hipMemsetAsync(counters, 0, 4, c->stream));
         HIPX(c, hipEventRecord(c->ev_a, c->stream));
-        int rc = launch_trace(c, sc, false, n, buf[cur].o, buf[cur].d, nullptr);
+        int rc = launch_trace(c, sc, false, n, buf[cur].o, buf[cur].d, nullptr, nullptr, nullptr, nullptr, true);
         if (rc) return rc;
         HIPX(c, hipEventRecord(c->ev_b, c->stream));
         c->stats.launches_extend++;

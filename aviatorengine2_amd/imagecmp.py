@@ -7,7 +7,7 @@ is written from the definitions).
     rmse_rel(a, b)    sqrt(mean((a-b)^2)) / mean(b) over RGB (relative RMSE)
     max_abs(a, b)     largest per-channel absolute difference
 
-CLI:  python -m aviatorengine2_amd.imagecmp a.npy b.npy
+CLI:  python -m aviatorengine2_amd.imagecmp a b   (.npy, .png, .ppm, .pfm or .exr)
 """
 from __future__ import annotations
 
@@ -64,7 +64,8 @@ def main(argv):
     if len(argv) != 3:
         print(__doc__)
         return 2
-    a, b = np.load(argv[1], allow_pickle=False), np.load(argv[2], allow_pickle=False)
+    from .imagefile import load
+    a, b = load(argv[1]), load(argv[2])
     print(report(a, b))
     return 0
 

@@ -151,6 +151,17 @@ class Renderer:
         self._check(self.lib.avrt_read_display(self.ctx, ptr(out), out.nbytes), "avrt_read_display")
         return out
 
+    def SaveDisplay(self, path):
+        """The RGBA8 sRGB display as PNG (or .ppm / .npy by extension)."""
+        from .imagefile import save
+        d = self.ReadDisplay()
+        save(path, d[..., :3] if str(path).lower().endswith(".ppm") else d)
+
+    def SaveImage(self, path):
+        """The linear accumulation image as OpenEXR HALF (or .pfm / .npy)."""
+        from .imagefile import save
+        save(path, self.ReadImage())
+
     # -- tuning (traversal variant, bounce-ray sort, chunking) -------------
     def GetTuning(self) -> dict:
         t = abi.AvrtTuning()

@@ -50,7 +50,7 @@ def bytes_per_ray(counts):
     return (48 * nodes + 128 * objs + 48 * insts + 108 * tris) / rays + 64.0
 
 
-def measured_traffic(args):
+def measured_traffic(args, kernel):
     """HBM bytes per launch of the extension traversal, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (scaled
     by the copy-kernel calibration factors), when this run is that workload."""
@@ -60,6 +60,8 @@ def measured_traffic(args):
         return None
     with open(path) as f:
         t = json.load(f)
+    if t.get("kernel") != kernel:
+        return None
     return 1024.0 * (t["fetch_kib_per_launch"] * t["fetch_factor"] + t["write_kib_per_launch"] * t["write_factor"])
 
 
@@ -74,7 +76,7 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--tex", type=int, default=4096)
-    ap.add_argument("--cpu-stride", type=int, default=8,
+    ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -146,7 +148,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        threads = os.cpu_count() or 1
+        threads = min(16, os.cpu_count() or 1)
         log(f"CPU oracle: every {args.cpu_stride}th pixel of one frame on {threads} threads")
         _, _, secs, cnt = oracle.render(packed, cfg, abi.MODE_RAY_TRACE, threads=threads, stats=True,
                                         pixel_stride=args.cpu_stride)
@@ -155,15 +157,26 @@ def main():
         cpu = {"value": cpu_rays / secs / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"1/{args.cpu_stride} of the pixels of one {args.width}x{args.height}x{args.spp}spp frame "
                          f"({cpu_rays} rays, {secs:.1f} s)"}
+        # one core on a smaller sample of the same frame
+        stride1 = args.cpu_stride * 16
+        _, _, secs1, cnt1 = oracle.render(packed, cfg, abi.MODE_RAY_TRACE, threads=1, stats=True,
+                                          pixel_stride=stride1)
+        rays1 = int(cnt1[0] + cnt1[5])
+        cpu["single_core"] = {"value": rays1 / secs1 / 1e6, "unit": "Mrays/s",
+                              "sample": f"1/{stride1} of the pixels ({rays1} rays, {secs1:.1f} s)"}
+        log(f"CPU oracle: {cpu['value']:.2f} Mrays/s on {threads} threads, {cpu['single_core']['value']:.2f} on one")
         b_ext = bytes_per_ray(ext)
         ext_rays = tot["rays_primary"] + tot["rays_bounce"]
         launches = max(1, tot["launches_extend"])
         per_launch_bytes = b_ext * ext_rays / launches
         per_launch_ms = tot["ms_extend"] / launches
+        kernel = TRACE_KERNEL[sched]
+        if tuning is not None and sched == 0:
+            kernel = f"k_trace_ww<false, {tuning['best']['stack_lds']}, {tuning['best']['block']}>"
         achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                    "kernel": TRACE_KERNEL[sched],
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, kernel),
+                    "kernel": kernel,
                     "bytes_per_ray": b_ext, "avg_launch_ms": per_launch_ms}
 
     served = run_served(args, mrays) if args.served_jobs > 0 else None
