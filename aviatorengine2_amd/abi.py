@@ -130,7 +130,7 @@ JOB_QUEUED, JOB_ACTIVE, JOB_DONE = 0, 1, 2
 
 class AvrtTuning(ctypes.Structure):
     _fields_ = [("stack_lds", ctypes.c_uint32), ("block", ctypes.c_uint32), ("sort_bits", ctypes.c_uint32),
-                ("samples_per_chunk", ctypes.c_uint32)]
+                ("samples_per_chunk", ctypes.c_uint32), ("graph", ctypes.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

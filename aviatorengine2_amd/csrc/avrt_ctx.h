@@ -6,6 +6,7 @@
 
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/avrt.h"
@@ -74,6 +75,8 @@ struct HybridState {
 // Variants of the persistent pass walk (see kTraceVariants in avrt_render.hip).
 constexpr int kNumTraceVariants = 6;
 constexpr int kDefaultTraceVariant = 3; // 16-entry LDS stack, 256 threads
+// Deepest bounce loop that avrt_render captures as one graph.
+constexpr int kGraphMaxDepth = 16;
 
 struct avrt_ctx {
     int device = 0;
@@ -135,6 +138,16 @@ struct avrt_ctx {
     // a third set of ray buffers the sorted queue is gathered into
     avrt::DevBuf rs_key[2], rs_val[2], rs_tmp, rs_o, rs_d, rs_t;
     size_t cap_sort = 0, rs_tmp_bytes = 0;
+    // captured bounce loop: the instantiated graph and the launch shape it was
+    // captured for (kernel arguments, buffers, counts), the per-replay
+    // constants (device copy and pinned host staging), per-bounce counters,
+    // and timing events recorded by the graph itself
+    bool use_graph = true;
+    std::vector<std::pair<std::vector<unsigned char>, hipGraphExec_t>> graphs; // most recent last, at most 4
+    avrt::DevBuf d_gframe, gcounters;
+    void *h_gframe = nullptr;
+    hipEvent_t gev[kGraphMaxDepth][5] = {};
+    hipEvent_t gev_io[4] = {};
 
     avrt::SceneRebuild rb;
     avrt::HybridState hy;

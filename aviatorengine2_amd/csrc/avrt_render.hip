@@ -40,6 +40,8 @@
 using namespace avrt;
 using namespace avrt::dev;
 
+static void drop_graph(avrt_ctx *c);
+
 namespace {
 
 // Counter words: the claim counters of the two streams' trace launches at 0
@@ -143,6 +145,66 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams fp, uint32_t fir
     const uint32_t bq = blockIdx.x * blockDim.x + threadIdx.x; // band position of the pixel
     if (bq >= fp.width * fp.height) return;
     resolve_pixel<MODE>(fp, bq, first_sample, ns, Lpath, image, display);
+}
+
+// Kernels of the captured bounce loop. The frame constants and the chunk's
+// first sample come from device memory (GraphFrame, copied before each
+// replay) and every stage after the camera rays reads its ray count from the
+// counter the stage before it wrote, so one instantiated graph serves every
+// frame and chunk of the same shape. Shading and the NEE sums stride over
+// their queue with a fixed grid.
+struct GraphFrame {
+    FrameParams fp;
+    uint32_t s0;  // first sample frame of the chunk
+    uint32_t n0;  // camera paths of the chunk
+};
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t words, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen_g(const GraphFrame *__restrict__ g, uint32_t ns, uint32_t n,
+                                                     PathBufs out, float4 *Lpath)
+{
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= n) return;
+    raygen_path(g->fp, g->s0, ns, pid, pid, out, Lpath);
+}
+
+template <int MODE, bool WAVE_NEE>
+__global__ __launch_bounds__(kBlock) void k_shade_g(SceneDev sc, const GraphFrame *__restrict__ g,
+                                                    const uint32_t *__restrict__ n_dev, uint32_t bounce, PathBufs in,
+                                                    const uint4 *__restrict__ hitA, const float *__restrict__ hitB,
+                                                    PathBufs out, ShadowBufs sh, ShadowRays sr, NeeBufs nee,
+                                                    float4 *Lpath, uint32_t *counters)
+{
+    __shared__ float lut[256];
+    const float *l = srgb_to_lds(lut);
+    const uint32_t n = *n_dev;
+    const FrameParams fp = g->fp;
+    // block-uniform trip count: every lane of a wave takes part in the
+    // queue appends of each pass
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < n; k0 += gridDim.x * blockDim.x)
+        shade_path<MODE, WAVE_NEE>(sc, fp, k0 + threadIdx.x, n, bounce, in, hitA, hitB, out, sh, sr, nee, Lpath,
+                                   counters, l);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nee_finalize_g(const uint32_t *__restrict__ n_dev, uint32_t nl, NeeBufs nee,
+                                                           float4 *Lpath)
+{
+    const uint32_t n = *n_dev;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+        nee_finalize_record(r, nl, nee, Lpath);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_resolve_g(const GraphFrame *__restrict__ g, uint32_t ns,
+                                                      const float4 *Lpath, float4 *image, uchar4 *display)
+{
+    const uint32_t bq = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bq >= g->fp.width * g->fp.height) return;
+    resolve_pixel<MODE>(g->fp, bq, g->s0, ns, Lpath, image, display);
 }
 
 template <bool REF_ORDER>
@@ -457,6 +519,15 @@ int avrt_destroy(avrt_ctx *c)
                       &c->rs_val[1], &c->rs_tmp, &c->rs_o, &c->rs_d, &c->rs_t};
     for (DevBuf *b : bufs) release(*b);
     hybrid_release(c);
+    drop_graph(c);
+    release(c->d_gframe);
+    release(c->gcounters);
+    if (c->h_gframe) (void)hipHostFree(c->h_gframe);
+    for (auto &row : c->gev)
+        for (hipEvent_t e : row)
+            if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->gev_io)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {c->ev0, c->ev1, c->ev_a, c->ev_b, c->ev_s0, c->ev_s1, c->ev_shaded, c->ev_nee})
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -733,28 +804,31 @@ static uint32_t trace_grid(avrt_ctx *c, uint32_t n)
 template <bool SHADOW, int K, int B>
 static void launch_ww_kb(const avrt_ctx *c, int v, hipStream_t st, const SceneDev &sc, uint32_t n, const float4 *ro,
                          const float4 *rd, const float4 *rlp, uint4 *hA, float *hB, uint32_t *work, float4 *contrib,
-                         uint32_t *segs)
+                         uint32_t *segs, const uint32_t *n_dev)
 {
+    // with n_dev the count is known only on the device: the grid is every
+    // resident workgroup and n is an upper bound
     const uint32_t resident = static_cast<uint32_t>(c->cu_count * c->ww_blocks[v]);
-    const uint32_t blocks = std::max(1u, std::min((n + B - 1) / B, resident));
+    const uint32_t blocks = n_dev ? std::max(1u, resident) : std::max(1u, std::min((n + B - 1) / B, resident));
     const uint32_t waves = blocks * (B / 64);
     const uint32_t chunk = std::min(1024u, std::max(64u, (n / (16u * waves)) & ~63u));
     hipLaunchKernelGGL((k_trace_ww<SHADOW, K, B>), dim3(blocks), dim3(B), 0, st, sc, n, ro, rd, rlp, hA, hB, work, chunk,
-                       contrib, segs);
+                       contrib, segs, n_dev);
 }
 
 template <bool SHADOW>
 static void launch_ww(const avrt_ctx *c, hipStream_t st, const SceneDev &sc, uint32_t n, const float4 *ro, const float4 *rd,
-                      const float4 *rlp, uint4 *hA, float *hB, uint32_t *work, float4 *contrib, uint32_t *segs)
+                      const float4 *rlp, uint4 *hA, float *hB, uint32_t *work, float4 *contrib, uint32_t *segs,
+                      const uint32_t *n_dev = nullptr)
 {
     const int v = c->trace_variant;
     switch (v) {
-    case 0: launch_ww_kb<SHADOW, 8, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
-    case 1: launch_ww_kb<SHADOW, 8, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
-    case 2: launch_ww_kb<SHADOW, 16, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
-    case 4: launch_ww_kb<SHADOW, 32, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
-    case 5: launch_ww_kb<SHADOW, 32, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
-    default: launch_ww_kb<SHADOW, 16, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs); break;
+    case 0: launch_ww_kb<SHADOW, 8, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
+    case 1: launch_ww_kb<SHADOW, 8, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
+    case 2: launch_ww_kb<SHADOW, 16, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
+    case 4: launch_ww_kb<SHADOW, 32, 128>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
+    case 5: launch_ww_kb<SHADOW, 32, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
+    default: launch_ww_kb<SHADOW, 16, 256>(c, v, st, sc, n, ro, rd, rlp, hA, hB, work, contrib, segs, n_dev); break;
     }
 }
 
@@ -1011,6 +1085,204 @@ static int run_chunk(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t n
     return AVRT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Captured bounce loop. The same stages as run_chunk (with the fused shadow
+// walk on the second stream), recorded once into a hipGraph and replayed:
+// the counts each stage needs stay on the device, so a chunk is one graph
+// launch, one copy of its constants and one read of all counters at the end.
+// Counters are cleared by a fill kernel inside the graph, and each bounce has
+// its own counter slot. Timing events are event nodes of the graph.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGraphSlotWords = 6 * kCtr; // cont, records, shadow rays, segments, 2 claim counters
+
+static bool graph_eligible(const avrt_ctx *c, const FrameParams &fp, int mode)
+{
+    return c->use_graph && c->opts.trace_schedule == 0 && !c->ref_order && c->sort_bits < 3 && fp.depth >= 1 &&
+           fp.depth <= static_cast<uint32_t>(kGraphMaxDepth) &&
+           (mode == AVRT_MODE_PATH_TRACE || fp.num_lights <= static_cast<uint32_t>(kWaveLights));
+}
+
+struct GraphLaunch {
+    SceneDev sc;
+    PathBufs buf[2];
+    ShadowBufs sh;
+    ShadowRays sr;
+    NeeBufs nee;
+    float4 *Lpath;
+    float4 *image;
+    uchar4 *display;
+    GraphFrame *dg;
+    uint32_t *gc;
+    uint4 *hA;
+    float *hB;
+    uint64_t mode, P, ns, depth, nl, wave_nee, variant, resident, cap_shadow, cu;
+};
+
+template <int MODE, bool WAVE_NEE>
+static int record_bounce_loop(avrt_ctx *c, const GraphLaunch &g)
+{
+    const hipStream_t s1 = c->stream, s2 = c->stream2;
+    const uint32_t n = static_cast<uint32_t>(g.P * g.ns);
+    const uint32_t words = static_cast<uint32_t>(g.depth) * kGraphSlotWords;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(blocks_for(words), 64u)), dim3(kBlock), 0, s1, g.gc, words, 0u);
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipEventRecordWithFlags(c->gev_io[0], s1, hipEventRecordExternal));
+    hipLaunchKernelGGL(k_raygen_g, dim3(blocks_for(n)), dim3(kBlock), 0, s1, g.dg, static_cast<uint32_t>(g.ns), n, g.buf[0],
+                       g.Lpath);
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipEventRecordWithFlags(c->gev_io[1], s1, hipEventRecordExternal));
+    const uint32_t shade_blocks = std::min(blocks_for(n), static_cast<uint32_t>(g.cu) * 8u);
+    const uint32_t fin_blocks = std::max(1u, std::min(blocks_for(static_cast<uint32_t>(g.cap_shadow)), static_cast<uint32_t>(g.cu) * 8u));
+    int cur = 0;
+    for (uint32_t b = 0; b < g.depth; b++) {
+        uint32_t *slot = g.gc + b * kGraphSlotWords;
+        const uint32_t *n_dev = b == 0 ? &g.dg->n0 : g.gc + (b - 1) * kGraphSlotWords;
+        HIPX(c, hipEventRecordWithFlags(c->gev[b][0], s1, hipEventRecordExternal));
+        launch_ww<false>(c, s1, g.sc, n, g.buf[cur].o, g.buf[cur].d, nullptr, g.hA, g.hB, slot + 4 * kCtr, nullptr, nullptr,
+                         n_dev);
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipEventRecordWithFlags(c->gev[b][1], s1, hipEventRecordExternal));
+        // the NEE sums of bounce b - 1 land before this shade adds to the same paths
+        if (WAVE_NEE && b > 0) HIPX(c, hipStreamWaitEvent(s1, c->ev_nee, 0));
+        hipLaunchKernelGGL((k_shade_g<MODE, WAVE_NEE>), dim3(shade_blocks), dim3(kBlock), 0, s1, g.sc, g.dg, n_dev, b,
+                           g.buf[cur], g.hA, g.hB, g.buf[cur ^ 1], g.sh, g.sr, g.nee, g.Lpath, slot);
+        HIPX(c, hipGetLastError());
+        HIPX(c, hipEventRecordWithFlags(c->gev[b][2], s1, hipEventRecordExternal));
+        if (WAVE_NEE) {
+            HIPX(c, hipEventRecord(c->ev_shaded, s1));
+            HIPX(c, hipStreamWaitEvent(s2, c->ev_shaded, 0));
+            HIPX(c, hipEventRecordWithFlags(c->gev[b][3], s2, hipEventRecordExternal));
+            launch_ww<true>(c, s2, g.sc, static_cast<uint32_t>(g.cap_shadow), g.sr.o, g.sr.d, g.sr.lp, g.hA, g.hB,
+                            slot + 5 * kCtr, g.nee.contrib, slot + 3 * kCtr, slot + 2 * kCtr);
+            HIPX(c, hipGetLastError());
+            hipLaunchKernelGGL(k_nee_finalize_g, dim3(fin_blocks), dim3(kBlock), 0, s2, slot + kCtr,
+                               static_cast<uint32_t>(g.nl), g.nee, g.Lpath);
+            HIPX(c, hipGetLastError());
+            HIPX(c, hipEventRecordWithFlags(c->gev[b][4], s2, hipEventRecordExternal));
+            HIPX(c, hipEventRecord(c->ev_nee, s2));
+        }
+        cur ^= 1;
+    }
+    if (WAVE_NEE) HIPX(c, hipStreamWaitEvent(s1, c->ev_nee, 0));
+    HIPX(c, hipEventRecordWithFlags(c->gev_io[2], s1, hipEventRecordExternal));
+    hipLaunchKernelGGL(k_resolve_g<MODE>, dim3(blocks_for(static_cast<uint32_t>(g.P))), dim3(kBlock), 0, s1, g.dg,
+                       static_cast<uint32_t>(g.ns), g.Lpath, g.image, g.display);
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipEventRecordWithFlags(c->gev_io[3], s1, hipEventRecordExternal));
+    return AVRT_OK;
+}
+
+static void drop_graph(avrt_ctx *c)
+{
+    for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
+
+template <int MODE>
+static int run_chunk_graph(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns)
+{
+    int rc;
+    if ((rc = ensure(c, c->d_gframe, sizeof(GraphFrame))) ||
+        (rc = ensure(c, c->gcounters, static_cast<size_t>(kGraphMaxDepth) * kGraphSlotWords * sizeof(uint32_t))))
+        return rc;
+    if (c->h_gframe == nullptr) HIPX(c, hipHostMalloc(&c->h_gframe, sizeof(GraphFrame), hipHostMallocDefault));
+    for (auto &row : c->gev)
+        for (hipEvent_t &e : row)
+            if (e == nullptr) HIPX(c, hipEventCreate(&e));
+    for (hipEvent_t &e : c->gev_io)
+        if (e == nullptr) HIPX(c, hipEventCreate(&e));
+
+    GraphLaunch g;
+    std::memset(&g, 0, sizeof(g));
+    g.sc = scene_dev(c);
+    for (int i = 0; i < 2; i++)
+        g.buf[i] = PathBufs{static_cast<float4 *>(c->so[i].p), static_cast<float4 *>(c->sd[i].p), static_cast<float4 *>(c->st[i].p)};
+    g.sh = ShadowBufs{static_cast<float4 *>(c->shp.p), static_cast<float4 *>(c->shn.p), static_cast<float4 *>(c->sha.p),
+                      static_cast<float4 *>(c->shw.p)};
+    g.sr = ShadowRays{static_cast<float4 *>(c->sro[0].p), static_cast<float4 *>(c->srd[0].p), static_cast<float4 *>(c->srl[0].p)};
+    g.nee = NeeBufs{static_cast<float4 *>(c->nee_tp.p), static_cast<float4 *>(c->nee_contrib.p)};
+    g.Lpath = static_cast<float4 *>(c->Lpath.p);
+    g.image = static_cast<float4 *>(c->d_image.p);
+    g.display = static_cast<uchar4 *>(c->d_display.p);
+    g.dg = c->d_gframe.as<GraphFrame>();
+    g.gc = c->gcounters.as<uint32_t>();
+    g.hA = static_cast<uint4 *>(c->hitA.p);
+    g.hB = static_cast<float *>(c->hitB.p);
+    const bool wave_nee = (MODE == 1) && fp.num_lights > 0 && fp.num_lights <= kWaveLights;
+    g.mode = MODE;
+    g.P = static_cast<uint64_t>(fp.width) * fp.height;
+    g.ns = ns;
+    g.depth = fp.depth;
+    g.nl = fp.num_lights;
+    g.wave_nee = wave_nee;
+    g.variant = static_cast<uint64_t>(c->trace_variant);
+    g.resident = static_cast<uint64_t>(c->ww_blocks[c->trace_variant]);
+    g.cap_shadow = c->cap_shadow;
+    g.cu = static_cast<uint64_t>(c->cu_count);
+    const unsigned char *kb = reinterpret_cast<const unsigned char *>(&g);
+    std::vector<unsigned char> key(kb, kb + sizeof(g));
+    // a few shapes stay instantiated (a frame's chunks differ in the last one)
+    hipGraphExec_t exec = nullptr;
+    for (size_t i = 0; i < c->graphs.size(); i++)
+        if (c->graphs[i].first == key) {
+            exec = c->graphs[i].second;
+            std::rotate(c->graphs.begin() + i, c->graphs.begin() + i + 1, c->graphs.end());
+            break;
+        }
+    if (exec == nullptr) {
+        if (c->graphs.size() >= 4) {
+            (void)hipGraphExecDestroy(c->graphs.front().second);
+            c->graphs.erase(c->graphs.begin());
+        }
+        HIPX(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = wave_nee ? record_bounce_loop<MODE, true>(c, g) : record_bounce_loop<MODE, false>(c, g);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc == AVRT_OK && e != hipSuccess)
+            rc = set_err(c, AVRT_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (rc == AVRT_OK) {
+            const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e2 != hipSuccess) {
+                exec = nullptr;
+                rc = set_err(c, AVRT_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+            }
+        }
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        c->graphs.emplace_back(key, exec);
+    }
+    const uint32_t n = fp.width * fp.height * ns;
+    GraphFrame h;
+    std::memset(&h, 0, sizeof(h));
+    h.fp = fp;
+    h.s0 = s0;
+    h.n0 = n;
+    std::memcpy(c->h_gframe, &h, sizeof(h));
+    HIPX(c, hipMemcpyAsync(g.dg, c->h_gframe, sizeof(GraphFrame), hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipGraphLaunch(exec, c->stream));
+    std::vector<uint32_t> hc(static_cast<size_t>(fp.depth) * kGraphSlotWords);
+    HIPX(c, hipMemcpyAsync(hc.data(), g.gc, hc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+
+    c->stats.paths += n;
+    c->stats.rays_primary += n;
+    c->stats.ms_other += elapsed(c->gev_io[0], c->gev_io[1]) + elapsed(c->gev_io[2], c->gev_io[3]);
+    uint32_t nb = n;
+    for (uint32_t b = 0; b < fp.depth && nb > 0; b++) {
+        const uint32_t *w = hc.data() + static_cast<size_t>(b) * kGraphSlotWords;
+        if (b > 0) c->stats.rays_bounce += nb;
+        c->stats.launches_extend++;
+        c->stats.ms_extend += elapsed(c->gev[b][0], c->gev[b][1]);
+        c->stats.ms_shade += elapsed(c->gev[b][1], c->gev[b][2]);
+        if (wave_nee) {
+            c->stats.rays_shadow += w[2 * kCtr] + w[3 * kCtr];
+            c->stats.ms_shadow += elapsed(c->gev[b][3], c->gev[b][4]);
+        }
+        nb = w[0];
+    }
+    return AVRT_OK;
+}
+
 int avrt::frame_params(avrt_ctx *c, const AvrtConfig *cfg, FrameParams &fp)
 {
     std::memset(&fp, 0, sizeof(fp));
@@ -1063,7 +1335,10 @@ int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
     if (rc) return rc;
     for (uint32_t s0 = 0; s0 < S; s0 += per_chunk) {
         const uint32_t ns = std::min(per_chunk, S - s0);
-        rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk<0>(c, fp, s0, ns) : run_chunk<1>(c, fp, s0, ns);
+        if (graph_eligible(c, fp, mode))
+            rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk_graph<0>(c, fp, s0, ns) : run_chunk_graph<1>(c, fp, s0, ns);
+        else
+            rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk<0>(c, fp, s0, ns) : run_chunk<1>(c, fp, s0, ns);
         if (rc) return rc;
     }
     HIPX(c, hipEventRecord(c->ev1, c->stream));
@@ -1151,6 +1426,7 @@ int avrt_get_tuning(const avrt_ctx *c, AvrtTuning *out)
     out->block = kTraceVariants[c->trace_variant][1];
     out->sort_bits = c->sort_bits;
     out->samples_per_chunk = c->chunk_spp;
+    out->graph = c->use_graph ? 1u : 0u;
     return AVRT_OK;
 }
 
@@ -1163,7 +1439,9 @@ int avrt_set_tuning(avrt_ctx *c, const AvrtTuning *t)
     if (v < 0) return set_err(c, AVRT_E_INVALID_ARG, "stack_lds must be 8, 16 or 32 and block 128 or 256");
     if (t->sort_bits != 0 && (t->sort_bits < 3 || t->sort_bits > 32))
         return set_err(c, AVRT_E_INVALID_ARG, "sort_bits must be 0 or 3..32");
+    if (t->graph > 1) return set_err(c, AVRT_E_INVALID_ARG, "graph must be 0 or 1");
     c->trace_variant = v;
+    c->use_graph = t->graph != 0;
     c->sort_bits = t->sort_bits;
     c->chunk_spp = t->samples_per_chunk;
     return AVRT_OK;

@@ -463,8 +463,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(SceneDev sc, uint32_t n, con
                                                        const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
                                                        uint4 *__restrict__ hitA, float *__restrict__ hitB,
                                                        uint32_t *work, uint32_t chunk, float4 *__restrict__ contrib,
-                                                       uint32_t *segs)
+                                                       uint32_t *segs, const uint32_t *n_dev)
 {
+    // n_dev: the ray count left in device memory by the previous stage (a
+    // captured bounce loop); the claim chunk then follows from it as the host
+    // would compute it for a launch of this grid.
+    if (n_dev != nullptr) {
+        n = *n_dev;
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        chunk = min(1024u, max(64u, (n / (16u * waves)) & ~63u));
+    }
     __shared__ uint32_t lds_stack[K * BLOCK];
     uint32_t spill_mem[kStackSpill];
     Traversal<false, K, BLOCK, kStackSpill> tr;
@@ -598,10 +606,8 @@ struct NeeBufs {
 
 // Sum of the light contributions in light order, clamped to 8 and weighted by
 // the throughput (ColorRay, ray_trace.comp.hybrid.hlsl:1264-1274).
-__global__ __launch_bounds__(kBlock) void k_nee_finalize(uint32_t n, uint32_t nl, NeeBufs nee, float4 *Lpath)
+__device__ __forceinline__ void nee_finalize_record(uint32_t r, uint32_t nl, NeeBufs nee, float4 *Lpath)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
     F3 sum = f3(0.0f, 0.0f, 0.0f);
     for (uint32_t i = 0; i < nl; i++) {
         const float4 c = nee.contrib[r * nl + i];
@@ -611,6 +617,12 @@ __global__ __launch_bounds__(kBlock) void k_nee_finalize(uint32_t n, uint32_t nl
     if (luma > 8.0f) sum = sum * (8.0f / luma);
     const float4 T = nee.tp[r];
     add_radiance(Lpath, __float_as_uint(T.w), f3(T.x, T.y, T.z) * sum);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nee_finalize(uint32_t n, uint32_t nl, NeeBufs nee, float4 *Lpath)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) nee_finalize_record(r, nl, nee, Lpath);
 }
 
 

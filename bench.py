@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--tex", type=int, default=4096)
+    ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
+                    help="1: each chunk's bounce loop is one captured graph replay; 0: stage by stage")
     ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU oracle renders every N-th pixel of one frame (bounded sample)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -111,6 +113,8 @@ def main():
 
     sched = args.trace_schedule if args.trace_schedule is not None else (1 if args.lane_trace else 0)
     r = Renderer().Initialize(device=0, trace_schedule=sched)
+    if args.graph == 0:
+        r.SetTuning(**dict(r.GetTuning(), graph=0))
     r.UploadMeshGeometry(library[0], library[1], library[2])
     r.UploadTextures(spec.texture)
     r.SetAuthoring(objs, inst, local)
@@ -137,6 +141,7 @@ def main():
             tot[key] += s[key]
         log(f"step {k}: {s['ms_total']:.1f} ms, rays {s['rays_primary'] + s['rays_bounce'] + s['rays_shadow']}")
     wall = time.perf_counter() - t_start
+    in_use = r.GetTuning()
     r.Shutdown()
 
     rays = tot["rays_primary"] + tot["rays_bounce"] + tot["rays_shadow"]
@@ -204,7 +209,7 @@ def main():
                                                                 "ms_other")},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "tuning": tuning,
+        "tuning": {"in_use": in_use, "autotune": tuning},
         "served": served,
         "hybrid": hyb,
     }

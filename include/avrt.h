@@ -337,6 +337,12 @@ typedef struct AvrtTuning {
     uint32_t samples_per_chunk; /* sample frames traced together (the queue
                                    capacity in camera frames); 0 = as many as
                                    max_paths allows                              */
+    uint32_t graph;             /* 1 (default): each chunk's bounce loop is one
+                                   captured hipGraph replay, ray counts stay on
+                                   the device; 0: stage by stage with a host
+                                   read of the counts after each shade. The
+                                   graph needs sort_bits 0 and depth <= 16 and
+                                   falls back to stages otherwise              */
 } AvrtTuning;
 
 typedef struct AvrtTuneReport {

@@ -226,18 +226,18 @@ def test_tuning_settings_give_identical_images(c3):
     cfg = small_cfg(c3.config, 160, 90, 4)
     with Renderer().Initialize() as r:
         r.LoadScene(c3)
-        assert r.GetTuning() == {"stack_lds": 16, "block": 256, "sort_bits": 0, "samples_per_chunk": 0}
+        assert r.GetTuning() == {"stack_lds": 16, "block": 256, "sort_bits": 0, "samples_per_chunk": 0, "graph": 1}
         ref_counts = _counts(r.RenderFrame(cfg, abi.MODE_RAY_TRACE))
         ref = r.ReadImage()
         for k in (8, 16, 32):
             for blk in (128, 256):
-                for bits, chunk in ((0, 0), (12, 2), (32, 1)):
-                    say(f"tuning stack {k} block {blk} sort {bits} chunk {chunk}")
-                    r.SetTuning(k, blk, bits, chunk)
+                for bits, chunk, graph in ((0, 0, 1), (0, 0, 0), (0, 3, 1), (12, 2, 1), (32, 1, 0)):
+                    say(f"tuning stack {k} block {blk} sort {bits} chunk {chunk} graph {graph}")
+                    r.SetTuning(k, blk, bits, chunk, graph)
                     counts = _counts(r.RenderFrame(cfg, abi.MODE_RAY_TRACE))
-                    assert counts == ref_counts, (k, blk, bits, chunk)
-                    assert np.array_equal(r.ReadImage(), ref), (k, blk, bits, chunk)
-        for bad in ((12, 256, 0, 0), (16, 64, 0, 0), (16, 256, 2, 0), (16, 256, 33, 0)):
+                    assert counts == ref_counts, (k, blk, bits, chunk, graph)
+                    assert np.array_equal(r.ReadImage(), ref), (k, blk, bits, chunk, graph)
+        for bad in ((12, 256, 0, 0), (16, 64, 0, 0), (16, 256, 2, 0), (16, 256, 33, 0), (16, 256, 0, 0, 2)):
             with pytest.raises(AvrtError):
                 r.SetTuning(*bad)
 

@@ -194,3 +194,58 @@ def test_frames_are_deterministic(sched):
     for f, (a, b) in enumerate(zip(*outs)):
         assert a[1:] == b[1:], (f, a[1:], b[1:])
         assert np.array_equal(a[0], b[0]), f
+
+
+@pytest.mark.parametrize("chunk", [0, 3])
+def test_graph_replays_equal_stage_by_stage(chunk):
+    """The captured bounce loop against the stage-by-stage loop, frame by
+    frame: the TLAS moves and the camera turns between frames (new constants
+    for each replay, with eager work and host reads in between), and with
+    chunk = 3 a 4-spp frame replays two graph shapes. Images and ray counts
+    are identical on every replay."""
+    spec = scenes.instanced(n_instances=256, tex_size=256, width=320, height=180, spp=4, depth=8)
+    builder = scenes.Builder()
+    library = scenes.mesh_library(spec.meshes, builder)
+    packed = scenes.pack(spec, builder, library)
+    objs, inst, local = scenes.authoring(spec, library)
+    outs = []
+    for graph in (1, 0):
+        with Renderer().Initialize() as r:
+            r.SetTuning(samples_per_chunk=chunk, graph=graph)
+            r.UploadMeshGeometry(library[0], library[1], library[2])
+            r.UploadTextures(spec.texture)
+            r.SetAuthoring(objs, inst, local)
+            frames = []
+            for f in range(4):
+                print(f"[gpu] graph {graph} chunk {chunk} frame {f}", flush=True)
+                cfg = packed.config.copy()
+                cfg["source_x"] = cfg["source_x"] + np.float32(3.0 * f)
+                r.SceneStep(spec.velocities, 1.0 / 60.0)
+                st = r.RenderFrame(cfg, abi.MODE_RAY_TRACE)
+                frames.append((r.ReadImage(), st["rays_primary"], st["rays_bounce"], st["rays_shadow"],
+                               st["launches_extend"]))
+        outs.append(frames)
+    for f, (a, b) in enumerate(zip(*outs)):
+        assert a[1:] == b[1:], (f, a[1:], b[1:])
+        assert np.array_equal(a[0], b[0]), f
+
+
+def test_graph_path_trace_accumulates_like_stages():
+    """path_trace semantics (no NEE stream) through the captured loop:
+    progressive accumulation over batches equals the stage-by-stage loop."""
+    packed = scenes.pack(scenes.sphere_field(width=160, height=90, spp=2, depth=6))
+    outs = []
+    for graph in (1, 0):
+        with Renderer().Initialize() as r:
+            r.SetTuning(graph=graph)
+            r.LoadScene(packed)
+            imgs = []
+            for batch in range(3):
+                print(f"[gpu] path_trace graph {graph} batch {batch}", flush=True)
+                cfg = packed.config.copy()
+                cfg["batch"] = batch
+                r.RenderFrame(cfg, abi.MODE_PATH_TRACE)
+                imgs.append(r.ReadImage())
+        outs.append(imgs)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
