@@ -222,6 +222,7 @@ __device__ void wg_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t 
 
 struct BuildWork {
     uint32_t *idx, *idx2, *flag, *flag2, *scanA, *scanB, *posL, *posR, *fbeg;
+    uint32_t *ftask; // level-local: the task of each flat position
     float *cen;
     uint32_t *t_start, *t_end, *t_depth, *t_left, *t_right, *t_size, *t_pre, *t_leaf, *t_mid, *t_part, *t_kids;
     int32_t *t_axis, *t_split;
@@ -294,6 +295,8 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         __syncthreads();
         wg_scan(w.flag, L, w.fbeg, sh_wave, &sh_carry);
         const uint32_t F = w.fbeg[L];
+        // the task of each flat position, searched once per level
+        for (uint32_t f = tid; f < F; f += kBT) w.ftask[f] = find_task(w.fbeg, L, f);
         for (uint32_t k = tid; k < L; k += kBT) {
             const uint32_t t = lb + k;
             for (int c = 0; c < 3; c++) {
@@ -304,22 +307,49 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
             }
         }
         __syncthreads();
-        // node box (first-in-index-order among equal values) and centroid box
-        for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
-            const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
-            const uint32_t prim = w.idx[pos];
-            const AvrtAABB b = bounds[prim];
-            atomicMin(&w.t_key[6 * t + 0], key_min(b.min_x, pos));
-            atomicMin(&w.t_key[6 * t + 1], key_min(b.min_y, pos));
-            atomicMin(&w.t_key[6 * t + 2], key_min(b.min_z, pos));
-            atomicMax(&w.t_key[6 * t + 3], key_max(b.max_x, pos));
-            atomicMax(&w.t_key[6 * t + 4], key_max(b.max_y, pos));
-            atomicMax(&w.t_key[6 * t + 5], key_max(b.max_z, pos));
-            for (int c = 0; c < 3; c++) {
-                const uint32_t s = f2sort(w.cen[3 * prim + c]);
-                atomicMin(&w.t_cbox[6 * t + c], s);
-                atomicMax(&w.t_cbox[6 * t + 3 + c], s);
+        // node box (first-in-index-order among equal values) and centroid box;
+        // a wave whose positions all belong to one task reduces its keys
+        // first and updates the task with one lane (min and max are
+        // associative, so the result is the same)
+        for (uint32_t base = tid & ~63u; base < F; base += kBT) {
+            const uint32_t f = base + (tid & 63u);
+            const bool active = f < F;
+            const uint32_t k = active ? w.ftask[f] : 0u, t = lb + k;
+            unsigned long long key[6];
+            uint32_t cb[6];
+            if (active) {
+                const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
+                const uint32_t prim = w.idx[pos];
+                const AvrtAABB b = bounds[prim];
+                key[0] = key_min(b.min_x, pos);
+                key[1] = key_min(b.min_y, pos);
+                key[2] = key_min(b.min_z, pos);
+                key[3] = key_max(b.max_x, pos);
+                key[4] = key_max(b.max_y, pos);
+                key[5] = key_max(b.max_z, pos);
+                for (int c = 0; c < 3; c++) cb[c] = cb[3 + c] = f2sort(w.cen[3 * prim + c]);
+            }
+            const uint32_t k0 = __shfl(k, 0);
+            const bool uniform = __all(active && k == k0);
+            if (uniform) {
+                for (int off = 32; off > 0; off >>= 1) {
+                    for (int c = 0; c < 3; c++) {
+                        const unsigned long long a = __shfl_xor(key[c], off), bmax = __shfl_xor(key[3 + c], off);
+                        key[c] = a < key[c] ? a : key[c];
+                        key[3 + c] = bmax > key[3 + c] ? bmax : key[3 + c];
+                        const uint32_t u = __shfl_xor(cb[c], off), v = __shfl_xor(cb[3 + c], off);
+                        cb[c] = u < cb[c] ? u : cb[c];
+                        cb[3 + c] = v > cb[3 + c] ? v : cb[3 + c];
+                    }
+                }
+            }
+            if (active && (!uniform || (tid & 63u) == 0u)) {
+                for (int c = 0; c < 3; c++) {
+                    atomicMin(&w.t_key[6 * t + c], key[c]);
+                    atomicMax(&w.t_key[6 * t + 3 + c], key[3 + c]);
+                    atomicMin(&w.t_cbox[6 * t + c], cb[c]);
+                    atomicMax(&w.t_cbox[6 * t + 3 + c], cb[3 + c]);
+                }
             }
         }
         __syncthreads();
@@ -351,7 +381,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         __syncthreads();
         // 12-bin histograms per axis
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             if (w.t_leaf[t]) continue;
             const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
             const uint32_t prim = w.idx[pos];
@@ -419,7 +449,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         __syncthreads();
         // predicate of std::partition per position
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             uint32_t p = 0;
             if (!w.t_leaf[t] && w.t_axis[t] >= 0) {
                 const int a = w.t_axis[t];
@@ -450,7 +480,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         // misplaced on the left (pred false below mid) and on the right (pred
         // true at or above mid)
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
             const uint32_t pr = w.flag2[f];
             const bool on = w.t_part[t] != 0;
@@ -461,7 +491,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         wg_scan(w.flag, F, w.scanA, sh_wave, &sh_carry);
         wg_scan(w.flag2, F, w.scanB, sh_wave, &sh_carry);
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
             const uint32_t base = w.fbeg[k];
             if (w.flag[f]) w.posL[base + (w.scanA[f] - w.scanA[base])] = pos;
@@ -472,7 +502,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         }
         __syncthreads();
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
             const uint32_t base = w.fbeg[k];
             const uint32_t v = w.idx[pos];
@@ -492,7 +522,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
         }
         __syncthreads();
         for (uint32_t f = tid; f < F; f += kBT) {
-            const uint32_t k = find_task(w.fbeg, L, f), t = lb + k;
+            const uint32_t k = w.ftask[f], t = lb + k;
             const uint32_t pos = w.t_start[t] + (f - w.fbeg[k]);
             w.idx[pos] = w.idx2[pos];
         }
@@ -806,6 +836,7 @@ BuildWork carve(void *base, uint32_t N, size_t *bytes_out)
     w.posL = static_cast<uint32_t *>(take(4 * n));
     w.posR = static_cast<uint32_t *>(take(4 * n));
     w.fbeg = static_cast<uint32_t *>(take(4 * (n + 1)));
+    w.ftask = static_cast<uint32_t *>(take(4 * (n + 1)));
     w.cen = static_cast<float *>(take(12 * n));
     w.t_start = static_cast<uint32_t *>(take(4 * T));
     w.t_end = static_cast<uint32_t *>(take(4 * T));
