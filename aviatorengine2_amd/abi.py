@@ -107,7 +107,7 @@ class AvrtStats(ctypes.Structure):
 
 class AvrtServerOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("max_jobs", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
-                ("round_paths", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("round_paths", ctypes.c_uint32), ("trace_schedule", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class AvrtJobStatus(ctypes.Structure):

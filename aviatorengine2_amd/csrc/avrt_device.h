@@ -278,6 +278,16 @@ struct HitRec {
     float alpha, beta;
 };
 
+// A pointer the whole wave shares, as a value in scalar registers.
+template <typename T>
+__device__ __forceinline__ const T *uniform_ptr(const T *p)
+{
+    const uint64_t u = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+    return reinterpret_cast<const T *>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 struct SceneDev {
     const Node4 *tlas;       // super-root at 0 (null when the scene is empty)
     const ObjGeom *geom;     // per TLAS slot
@@ -551,7 +561,11 @@ struct Traversal {
     __device__ __forceinline__ bool interior(const SceneDev &sc)
     {
         {
-            const Node4 *np = (in_blas ? sc.blas : sc.tlas) + ref;
+            // both tree pointers as wave-uniform values, then a per-lane
+            // select: the compiler would otherwise turn the select of the two
+            // kernel arguments into a per-lane load of the argument block,
+            // one more dependent load per node visit
+            const Node4 *np = (in_blas ? uniform_ptr(sc.blas) : uniform_ptr(sc.tlas)) + ref;
             const float4 LX = *reinterpret_cast<const float4 *>(np->lox);
             const float4 HX = *reinterpret_cast<const float4 *>(np->hix);
             const float4 LY = *reinterpret_cast<const float4 *>(np->loy);

@@ -90,8 +90,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_raygen(SegLaunch L, const JobDev
     raygen_path(J.fp, J.first_sample, J.ns, local, J.path_base + local, out, Lpath);
 }
 
-// Closest-hit queries of the jobs' queues (extension rays, or shadow rays
-// bounded by their light points).
+// One ray per lane (server option trace_schedule = 0): closest-hit queries
+// of the jobs' queues (extension rays, or shadow rays bounded by their light
+// points).
 template <bool SHADOW>
 __global__ __launch_bounds__(kBlock) void k_seg_trace(SegLaunch L, const JobDev *__restrict__ jobs,
                                                       const float4 *__restrict__ ro, const float4 *__restrict__ rd,
@@ -120,6 +121,32 @@ __global__ __launch_bounds__(kBlock) void k_seg_trace(SegLaunch L, const JobDev 
     }
     hitA[idx] = make_uint4(__float_as_uint(tr.hit.t), tr.hit.obj, tr.hit.prim, __float_as_uint(tr.hit.alpha));
     hitB[idx] = tr.hit.beta;
+}
+
+// The persistent pass walk over every job's queue of one bounce: each
+// workgroup serves one job (its scene stays in scalar registers), claims
+// that job's rays in chunks from the job's own counter, and a shadow launch
+// resolves glass continuations and occlusion in place (continuation
+// segments counted per job over the round).
+constexpr uint32_t kWorkBase = 4u * kCtr * kMaxRoundJobs;        // claim counter of job j: kWorkBase + j * kCtr
+constexpr uint32_t kSegBase = kWorkBase + kCtr * kMaxRoundJobs;  // shadow segments of job j this round
+constexpr uint32_t kServeCtrWords = kSegBase + kCtr * kMaxRoundJobs;
+
+template <bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_seg_trace_ww(SegLaunch L, const JobDev *__restrict__ jobs,
+                                                         const float4 *__restrict__ ro, const float4 *__restrict__ rd,
+                                                         const float4 *__restrict__ rlp, uint4 *__restrict__ hitA,
+                                                         float *__restrict__ hitB, float4 *contrib, uint32_t *ctr)
+{
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
+    const uint32_t j = job_of(L, blockIdx.x);
+    const uint32_t n = L.count[j];
+    const uint32_t base = (SHADOW ? 4u : 1u) * jobs[j].path_base;
+    const uint32_t waves = (L.blk[j + 1] - L.blk[j]) * (kBlock / 64);
+    const uint32_t chunk = min(1024u, max(64u, (n / (16u * waves)) & ~63u));
+    trace_ww_run<SHADOW, kStackLds, kBlock>(jobs[j].sc, n, ro + base, rd + base, SHADOW ? rlp + base : nullptr,
+                                            hitA + base, hitB + base, ctr + kWorkBase + j * kCtr, chunk,
+                                            SHADOW ? contrib + base : nullptr, ctr + kSegBase + j * kCtr, lds_stack);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev *__restrict__ jobs, uint32_t bounce,
@@ -216,6 +243,7 @@ struct ServeJob {
 struct avrt_server {
     AvrtServerOptions opts{};
     hipStream_t stream = nullptr;
+    uint32_t resident[2] = {256, 256}; // workgroups of the persistent walk (extension, shadow) the device holds at once
     std::string err;
     std::vector<ServeScene> scenes;
     std::unordered_map<uint64_t, ServeJob> jobs;
@@ -298,7 +326,7 @@ int ensure_round(avrt_server *s, size_t paths)
     }
     if ((rc = salloc(s, s->hitA, 4 * paths * 16)) || (rc = salloc(s, s->hitB, 4 * paths * 4)) ||
         (rc = salloc(s, s->Lpath, paths * 16)) || (rc = salloc(s, s->nee_tp, paths * 16)) ||
-        (rc = salloc(s, s->nee_contrib, 4 * paths * 16)) || (rc = salloc(s, s->counters, kMaxRoundJobs * 16 * kCtr)) ||
+        (rc = salloc(s, s->nee_contrib, 4 * paths * 16)) || (rc = salloc(s, s->counters, kServeCtrWords * sizeof(uint32_t))) ||
         (rc = salloc(s, s->jobtab, kMaxRoundJobs * sizeof(JobDev))))
         return rc;
     s->round_cap = paths;
@@ -314,6 +342,28 @@ uint32_t make_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
         L.blk[j] = b;
         L.count[j] = count[j];
         b += (count[j] + kBlock - 1) / kBlock;
+    }
+    L.blk[n] = b;
+    return b;
+}
+
+// Launch table of the persistent walk: the resident workgroups, shared among
+// the jobs in proportion to their rays (at least one each, at most one per
+// kBlock rays); returns the grid. (Four times as many workgroups, so that
+// those of cheap jobs' rays free room for the others, measured slower: 2.2-2.3
+// against 2.8 Grays/s on the mixed stream.)
+uint32_t make_persistent_launch(SegLaunch &L, const uint32_t *count, uint32_t n, uint32_t resident)
+{
+    uint64_t tot = 0;
+    for (uint32_t j = 0; j < n; j++) tot += count[j];
+    L.njobs = n;
+    uint32_t b = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        L.blk[j] = b;
+        L.count[j] = count[j];
+        const uint32_t dense = (count[j] + kBlock - 1) / kBlock;
+        const uint32_t share = tot ? static_cast<uint32_t>((static_cast<uint64_t>(resident) * count[j] + tot - 1) / tot) : 0u;
+        b += std::max(1u, std::min(dense, share));
     }
     L.blk[n] = b;
     return b;
@@ -364,6 +414,7 @@ int avrt_server_create(const AvrtServerOptions *opts, avrt_server **out)
     if (opts) o = *opts;
     if (o.hbm_budget_bytes == 0) o.hbm_budget_bytes = 200ull << 30;
     if (o.max_jobs == 0 || o.max_jobs > kMaxRoundJobs) o.max_jobs = kMaxRoundJobs;
+    if (o.trace_schedule > 1) return AVRT_E_INVALID_ARG;
     if (o.round_paths == 0) o.round_paths = 1u << 24;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return AVRT_E_DEVICE;
@@ -382,6 +433,15 @@ int avrt_server_create(const AvrtServerOptions *opts, avrt_server **out)
         avrt_server_destroy(s);
         return AVRT_E_DEVICE;
     }
+    hipDeviceProp_t prop;
+    const int cu = (hipGetDeviceProperties(&prop, o.device) == hipSuccess && prop.multiProcessorCount > 0)
+                       ? prop.multiProcessorCount
+                       : 256;
+    int nb0 = 0, nb1 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb0, k_seg_trace_ww<false>, kBlock, 0) != hipSuccess || nb0 <= 0) nb0 = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb1, k_seg_trace_ww<true>, kBlock, 0) != hipSuccess || nb1 <= 0) nb1 = 1;
+    s->resident[0] = static_cast<uint32_t>(cu * nb0);
+    s->resident[1] = static_cast<uint32_t>(cu * nb1);
     // the round queues are sized once, for round_paths paths, and charged to
     // the budget before any scene or job
     const int rc = ensure_round(s, o.round_paths);
@@ -508,6 +568,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
     float4 *Lpath = s->Lpath.as<float4>();
     uint32_t *ctr = s->counters.as<uint32_t>();
     SegLaunch L;
+    SHIPX(s, hipMemsetAsync(ctr + kSegBase, 0, sizeof(uint32_t) * kCtr * J, s->stream));
     uint32_t grid = make_launch(L, npaths.data(), J);
     hipLaunchKernelGGL(k_seg_raygen, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[0], Lpath);
     SHIPX(s, hipGetLastError());
@@ -521,26 +582,48 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
         if (tot == 0) break;
         if (b == 0) s->stats.rays_primary += tot;
         else s->stats.rays_bounce += tot;
-        grid = make_launch(L, cnt.data(), J);
-        hipLaunchKernelGGL((k_seg_trace<false>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[cur].o, buf[cur].d,
-                           static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>());
+        uint64_t any_rec = 0;
+        if (s->opts.trace_schedule == 1) {
+            SegLaunch Lt;
+            grid = make_persistent_launch(Lt, cnt.data(), J, s->resident[0]);
+            SHIPX(s, hipMemsetAsync(ctr + kWorkBase, 0, sizeof(uint32_t) * kCtr * J, s->stream));
+            hipLaunchKernelGGL((k_seg_trace_ww<false>), dim3(grid), dim3(kBlock), 0, s->stream, Lt, jobs, buf[cur].o,
+                               buf[cur].d, static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>(),
+                               static_cast<float4 *>(nullptr), ctr);
+        } else {
+            grid = make_launch(L, cnt.data(), J);
+            hipLaunchKernelGGL((k_seg_trace<false>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[cur].o, buf[cur].d,
+                               static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>());
+        }
         SHIPX(s, hipGetLastError());
+        grid = make_launch(L, cnt.data(), J);
         SHIPX(s, hipMemsetAsync(ctr, 0, 16 * kCtr * J, s->stream));
         hipLaunchKernelGGL(k_seg_shade, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, b, buf[cur],
                            s->hitA.as<const uint4>(), s->hitB.as<const float>(), buf[cur ^ 1], sr[0], nee, Lpath, ctr);
         SHIPX(s, hipGetLastError());
         if ((rc = read_job_counters(s, J, hc.data()))) return rc;
-        uint64_t any_rec = 0;
+        uint64_t tsr = 0;
         for (uint32_t j = 0; j < J; j++) {
             cnt[j] = hc[4 * j];
             nrec[j] = hc[4 * j + 1];
             nsr[j] = hc[4 * j + 2];
             any_rec += nrec[j];
+            tsr += nsr[j];
         }
-        if (any_rec) {
+        if (any_rec && tsr && s->opts.trace_schedule == 1) {
+            // shadow rays with glass continuations and occlusion resolved in the walk
+            s->stats.rays_shadow += tsr;
+            SegLaunch Lt;
+            grid = make_persistent_launch(Lt, nsr.data(), J, s->resident[1]);
+            SHIPX(s, hipMemsetAsync(ctr + kWorkBase, 0, sizeof(uint32_t) * kCtr * J, s->stream));
+            hipLaunchKernelGGL((k_seg_trace_ww<true>), dim3(grid), dim3(kBlock), 0, s->stream, Lt, jobs, sr[0].o,
+                               sr[0].d, sr[0].lp, s->hitA.as<uint4>(), s->hitB.as<float>(), nee.contrib, ctr);
+            SHIPX(s, hipGetLastError());
+        } else if (any_rec) {
+            // closest hits, then the glass steps on the host's count (up to 4 segments)
             int q = 0;
             for (int step = 0; step < 4; step++) {
-                uint64_t tsr = 0;
+                tsr = 0;
                 for (uint32_t j = 0; j < J; j++) tsr += nsr[j];
                 if (tsr == 0) break;
                 s->stats.rays_shadow += tsr;
@@ -556,6 +639,8 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
                 for (uint32_t j = 0; j < J; j++) nsr[j] = hc[4 * j + 3];
                 q ^= 1;
             }
+        }
+        if (any_rec) {
             grid = make_launch(L, nrec.data(), J);
             hipLaunchKernelGGL(k_seg_nee_finalize, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, nee, Lpath);
             SHIPX(s, hipGetLastError());
@@ -567,7 +652,12 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
     grid = make_launch(L, pix.data(), J);
     hipLaunchKernelGGL(k_seg_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, Lpath);
     SHIPX(s, hipGetLastError());
-    SHIPX(s, hipStreamSynchronize(s->stream));
+    {
+        std::vector<uint32_t> segs(static_cast<size_t>(kCtr) * J);
+        SHIPX(s, hipMemcpyAsync(segs.data(), ctr + kSegBase, segs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+        SHIPX(s, hipStreamSynchronize(s->stream));
+        for (uint32_t j = 0; j < J; j++) s->stats.rays_shadow += segs[static_cast<size_t>(j) * kCtr];
+    }
     s->stats.rounds++;
     const double t = now_s(s);
     for (uint32_t j = 0; j < J; j++) {

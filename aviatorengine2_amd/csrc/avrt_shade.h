@@ -458,22 +458,15 @@ constexpr int kRefillMin = 8;
 
 // K: LDS stack ring entries per lane; BLOCK: threads per workgroup (both
 // picked by the autotuner among the instantiated values).
-template <bool SHADOW, int K = kStackLds, int BLOCK = kBlock>
-__global__ __launch_bounds__(BLOCK) void k_trace_ww(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
-                                                       const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
-                                                       uint4 *__restrict__ hitA, float *__restrict__ hitB,
-                                                       uint32_t *work, uint32_t chunk, float4 *__restrict__ contrib,
-                                                       uint32_t *segs, const uint32_t *n_dev)
+// The walk itself, over rays [0, n) of the given arrays (offset by the
+// caller to a job's region in the served form). lds_stack: K * BLOCK words.
+template <bool SHADOW, int K, int BLOCK>
+__device__ __forceinline__ void trace_ww_run(const SceneDev &sc, uint32_t n, const float4 *__restrict__ ro,
+                                             const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                             uint4 *__restrict__ hitA, float *__restrict__ hitB, uint32_t *work,
+                                             uint32_t chunk, float4 *__restrict__ contrib, uint32_t *segs,
+                                             uint32_t *lds_stack)
 {
-    // n_dev: the ray count left in device memory by the previous stage (a
-    // captured bounce loop); the claim chunk then follows from it as the host
-    // would compute it for a launch of this grid.
-    if (n_dev != nullptr) {
-        n = *n_dev;
-        const uint32_t waves = gridDim.x * (BLOCK / 64);
-        chunk = min(1024u, max(64u, (n / (16u * waves)) & ~63u));
-    }
-    __shared__ uint32_t lds_stack[K * BLOCK];
     uint32_t spill_mem[kStackSpill];
     Traversal<false, K, BLOCK, kStackSpill> tr;
     uint32_t idx = 0;
@@ -564,22 +557,42 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(SceneDev sc, uint32_t n, con
             } else if (need && drained) {
                 done = true; // the queue is drained
             }
-            continue;
-        }
-        if (idle) break; // nothing to claim and no lane holds a ray
-        const int nn = __popcll(m_node), nb = __popcll(m_lb), nt = __popcll(m_lt);
-        if (nn > 0 && nn >= nb + nt) {
-            if (can_node) tr.node_visit(sc);
-        } else if (nb >= nt) {
-            if (leaf_b) tr.leaf_phase(sc);
         } else {
-            if (leaf_t) tr.leaf_phase(sc);
+            // one path back to the loop head (a separate `continue` path makes
+            // the compiler copy the whole ray state at each back edge)
+            if (idle) break; // nothing to claim and no lane holds a ray
+            const int nn = __popcll(m_node), nb = __popcll(m_lb), nt = __popcll(m_lt);
+            if (nn > 0 && nn >= nb + nt) {
+                if (can_node) tr.node_visit(sc);
+            } else {
+                // one leaf pass: BLAS leaves when at least as many lanes hold one
+                if (nb >= nt ? leaf_b : leaf_t) tr.leaf_phase(sc);
+            }
         }
     }
     if (SHADOW) {
         const uint32_t total = wave_sum(nseg);
         if (__lane_id() == 0 && total != 0u) atomicAdd(segs, total);
     }
+}
+
+template <bool SHADOW, int K = kStackLds, int BLOCK = kBlock>
+__global__ __launch_bounds__(BLOCK) void k_trace_ww(SceneDev sc, uint32_t n, const float4 *__restrict__ ro,
+                                                       const float4 *__restrict__ rd, const float4 *__restrict__ rlp,
+                                                       uint4 *__restrict__ hitA, float *__restrict__ hitB,
+                                                       uint32_t *work, uint32_t chunk, float4 *__restrict__ contrib,
+                                                       uint32_t *segs, const uint32_t *n_dev)
+{
+    // n_dev: the ray count left in device memory by the previous stage (a
+    // captured bounce loop); the claim chunk then follows from it as the host
+    // would compute it for a launch of this grid.
+    if (n_dev != nullptr) {
+        n = *n_dev;
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        chunk = min(1024u, max(64u, (n / (16u * waves)) & ~63u));
+    }
+    __shared__ uint32_t lds_stack[K * BLOCK];
+    trace_ww_run<SHADOW, K, BLOCK>(sc, n, ro, rd, rlp, hitA, hitB, work, chunk, contrib, segs, lds_stack);
 }
 
 __device__ __forceinline__ void add_radiance(float4 *Lpath, uint32_t pid, F3 c)

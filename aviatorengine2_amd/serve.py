@@ -30,13 +30,16 @@ from .renderer import Renderer
 
 
 class Server:
-    def __init__(self, device=0, hbm_budget_gib=200.0, max_jobs=64, round_paths=1 << 24):
+    def __init__(self, device=0, hbm_budget_gib=200.0, max_jobs=64, round_paths=1 << 24, trace_schedule=0):
+        """trace_schedule: 0 one ray per lane, 1 the persistent pass walk per
+        job (shadow rays resolved in the walk); the same images either way."""
         self.lib = abi.load_library()
         o = AvrtServerOptions()
         o.device = device
         o.max_jobs = max_jobs
         o.hbm_budget_bytes = int(hbm_budget_gib * (1 << 30))
         o.round_paths = round_paths
+        o.trace_schedule = trace_schedule
         h = ctypes.c_void_p()
         rc = self.lib.avrt_server_create(ctypes.byref(o), ctypes.byref(h))
         if rc:

@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--instances", type=int, default=1024)
     ap.add_argument("--tex", type=int, default=4096)
+    ap.add_argument("--served-schedule", type=int, default=0, choices=(0, 1),
+                    help="served traversal: 0 one ray per lane, 1 persistent pass walk per job")
     ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
                     help="1: each chunk's bounce loop is one captured graph replay; 0: stage by stage")
     ap.add_argument("--cpu-stride", type=int, default=4,
@@ -232,7 +234,7 @@ def run_served(args, plain_mrays):
     scene_list = serve.mixed_scenes(8)
     log(f"served: 8 scenes built in {time.perf_counter() - t0:.1f}s")
     jobs = serve.poisson_jobs(args.served_jobs, rate_hz=1e9, seed=0x10B5)
-    with serve.Server() as srv:
+    with serve.Server(trace_schedule=args.served_schedule) as srv:
         ids = [srv.add_scene(pk) for pk, _ in scene_list]
         # warm the kernels once
         warm = serve.poisson_jobs(4, rate_hz=1e9, seed=7, max_spp=2)
@@ -254,6 +256,7 @@ def run_served(args, plain_mrays):
             "jobs": len(jobs), "jobs_per_s": cap_jobs, "wall_s": wall,
             "latency": {"rate_jobs_per_s": rate, "load": args.served_load, "p50_ms": p50, "p99_ms": p99,
                         "jobs": len(ljobs), "wall_s": lwall},
+            "schedule": "one ray per lane" if args.served_schedule == 0 else "persistent pass walk per job",
             "stream": "8 scenes (4 sphere fields path_trace, 4 icosphere triangle scenes ray_trace), "
                       "res {256^2,640x360,1280x720,1920x1080}, spp log-uniform [1,64], depth 4-8"}
 

@@ -53,31 +53,37 @@ def test_job_alone_equals_plain_render(scene_set):
 
 def test_jobs_in_a_mix_equal_plain_renders(scene_set):
     """Uneven jobs batched together; a small round capacity splits the
-    larger jobs' samples over several rounds."""
-    say("served mix vs avrt_render")
+    larger jobs' samples over several rounds. Both traversal schedules of the
+    server (one ray per lane, persistent pass walk per job) give the plain
+    renders' images and the same ray counts."""
     specs = [(0, 160, 90, 3, 5), (1, 200, 120, 2, 4), (2, 96, 64, 4, 8), (1, 64, 64, 7, 6), (0, 128, 72, 1, 4),
              (2, 180, 100, 3, 5)]
-    with serve.Server(round_paths=60000) as srv:
-        sids = [srv.add_scene(pk) for pk, _ in scene_set]
-        jobs = []
-        for sc, w, h, spp, depth in specs:
+    counts = []
+    for sched in (0, 1):
+        say(f"served mix vs avrt_render, server schedule {sched}")
+        with serve.Server(round_paths=60000, trace_schedule=sched) as srv:
+            sids = [srv.add_scene(pk) for pk, _ in scene_set]
+            jobs = []
+            for sc, w, h, spp, depth in specs:
+                pk, mode = scene_set[sc]
+                jobs.append(srv.submit(sids[sc], serve.job_config(pk, w, h, spp, depth), mode))
+            rounds = 0
+            while any(srv.status(j).state != abi.JOB_DONE for j in jobs):
+                srv.step()
+                rounds += 1
+                assert rounds < 200
+            st = srv.stats()
+            say(f"rounds {rounds}, stats {st}")
+            assert st["rounds"] >= 3
+            counts.append((st["rays_primary"], st["rays_bounce"], st["rays_shadow"], st["paths"]))
+            out = [srv.read(j) for j in jobs]
+        for (sc, w, h, spp, depth), (img, disp) in zip(specs, out):
             pk, mode = scene_set[sc]
-            jobs.append(srv.submit(sids[sc], serve.job_config(pk, w, h, spp, depth), mode))
-        rounds = 0
-        while any(srv.status(j).state != abi.JOB_DONE for j in jobs):
-            srv.step()
-            rounds += 1
-            assert rounds < 200
-        st = srv.stats()
-        say(f"rounds {rounds}, stats {st}")
-        assert st["rounds"] >= 3
-        out = [srv.read(j) for j in jobs]
-    for (sc, w, h, spp, depth), (img, disp) in zip(specs, out):
-        pk, mode = scene_set[sc]
-        ref, rdisp = plain(pk, serve.job_config(pk, w, h, spp, depth), mode)
-        assert np.array_equal(img, ref), (sc, w, h, spp, depth)
-        if mode == abi.MODE_RAY_TRACE:
-            assert np.array_equal(disp, rdisp)
+            ref, rdisp = plain(pk, serve.job_config(pk, w, h, spp, depth), mode)
+            assert np.array_equal(img, ref), (sched, sc, w, h, spp, depth)
+            if mode == abi.MODE_RAY_TRACE:
+                assert np.array_equal(disp, rdisp)
+    assert counts[0] == counts[1], counts
 
 
 def test_admission_waits_for_budget(scene_set):
