@@ -562,12 +562,12 @@ __device__ __forceinline__ void trace_ww_run(const SceneDev &sc, uint32_t n, con
             // the compiler copy the whole ray state at each back edge)
             if (idle) break; // nothing to claim and no lane holds a ray
             const int nn = __popcll(m_node), nb = __popcll(m_lb), nt = __popcll(m_lt);
-            if (nn > 0 && nn >= nb + nt) {
-                if (can_node) tr.node_visit(sc);
-            } else {
-                // one leaf pass: BLAS leaves when at least as many lanes hold one
-                if (nb >= nt ? leaf_b : leaf_t) tr.leaf_phase(sc);
-            }
+            // a node pass, else one leaf pass (BLAS leaves when at least as
+            // many lanes hold one); written as two masked steps rather than
+            // if/else, which keeps the ray state in place across the join
+            const bool node_pass = nn > 0 && nn >= nb + nt;
+            if (node_pass && can_node) tr.node_visit(sc);
+            if (!node_pass && (nb >= nt ? leaf_b : leaf_t)) tr.leaf_phase(sc);
         }
     }
     if (SHADOW) {
