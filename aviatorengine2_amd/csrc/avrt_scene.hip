@@ -39,6 +39,7 @@ using namespace avrt;
 namespace {
 
 constexpr int kBT = 1024; // build workgroup size
+constexpr uint32_t kLdsBinTasks = 32; // levels with at most this many tasks bin in LDS
 constexpr uint32_t kErrBox = 1u;
 constexpr uint32_t kErrLeaf = 2u;
 constexpr uint32_t kErrBuild = 4u;
@@ -244,6 +245,18 @@ __device__ __forceinline__ uint32_t find_task(const uint32_t *fbeg, uint32_t L, 
     return lo;
 }
 
+// One primitive into a bin: box (order-preserving keys) and count.
+__device__ __forceinline__ void bin_add(uint32_t *bn, const AvrtAABB &b)
+{
+    atomicMin(&bn[0], f2sort(b.min_x));
+    atomicMin(&bn[1], f2sort(b.min_y));
+    atomicMin(&bn[2], f2sort(b.min_z));
+    atomicMax(&bn[3], f2sort(b.max_x));
+    atomicMax(&bn[4], f2sort(b.max_y));
+    atomicMax(&bn[5], f2sort(b.max_z));
+    atomicAdd(&bn[6], 1u);
+}
+
 __device__ __forceinline__ float half_area(float lx, float ly, float lz, float hx, float hy, float hz)
 {
     const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
@@ -264,6 +277,9 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
     __shared__ uint32_t sh_carry;
     __shared__ uint32_t sh_L, sh_lb, sh_ntasks, sh_nlev;
     __shared__ uint32_t sh_levels[64];
+    // histograms of a level with few tasks (the top of the tree, where all
+    // positions hit the same few bins) accumulate in LDS
+    __shared__ uint32_t sh_bins[kLdsBinTasks * 3 * 12 * 7];
     const uint32_t tid = threadIdx.x;
     if (N == 0) {
         if (tid == 0) info[0] = 0;
@@ -367,6 +383,7 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
                 for (int a = 0; a < 3; a++) {
                     const float extent = sort2f(w.t_cbox[6 * t + 3 + a]) - sort2f(w.t_cbox[6 * t + a]);
                     w.t_scale[3 * t + a] = (extent <= 0.0f) ? 0.0f : 12.0f / extent;
+                    if (L <= kLdsBinTasks) continue; // cleared below, in LDS
                     uint32_t *bn = w.bins + (static_cast<size_t>(k) * 3 + a) * 12 * 7;
                     for (int b = 0; b < 12; b++) {
                         for (int c = 0; c < 3; c++) {
@@ -376,6 +393,12 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
                         bn[b * 7 + 6] = 0;
                     }
                 }
+            }
+        }
+        if (L <= kLdsBinTasks) {
+            for (uint32_t i = tid; i < L * 3u * 12u * 7u; i += kBT) {
+                const uint32_t c = i % 7u;
+                sh_bins[i] = c < 3u ? f2sort(FLT_MAX) : (c < 6u ? f2sort(-FLT_MAX) : 0u);
             }
         }
         __syncthreads();
@@ -390,14 +413,9 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
                 const float scale = w.t_scale[3 * t + a];
                 if (scale == 0.0f) continue;
                 const int bin = sah_bin(w.cen[3 * prim + a], sort2f(w.t_cbox[6 * t + a]), scale);
-                uint32_t *bn = w.bins + ((static_cast<size_t>(k) * 3 + a) * 12 + bin) * 7;
-                atomicMin(&bn[0], f2sort(b.min_x));
-                atomicMin(&bn[1], f2sort(b.min_y));
-                atomicMin(&bn[2], f2sort(b.min_z));
-                atomicMax(&bn[3], f2sort(b.max_x));
-                atomicMax(&bn[4], f2sort(b.max_y));
-                atomicMax(&bn[5], f2sort(b.max_z));
-                atomicAdd(&bn[6], 1u);
+                const size_t off = ((static_cast<size_t>(k) * 3 + a) * 12 + bin) * 7;
+                if (L <= kLdsBinTasks) bin_add(sh_bins + off, b);
+                else bin_add(w.bins + off, b);
             }
         }
         __syncthreads();
@@ -409,7 +427,8 @@ __global__ __launch_bounds__(kBT) void k_build_tlas(uint32_t N, const AvrtAABB *
             float best_cost = FLT_MAX;
             for (int a = 0; a < 3; a++) {
                 if (w.t_scale[3 * t + a] == 0.0f) continue;
-                const uint32_t *bn = w.bins + (static_cast<size_t>(k) * 3 + a) * 12 * 7;
+                const size_t off = (static_cast<size_t>(k) * 3 + a) * 12 * 7;
+                const uint32_t *bn = (L <= kLdsBinTasks) ? sh_bins + off : w.bins + off;
                 float lc[11];
                 int ln[11];
                 float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
