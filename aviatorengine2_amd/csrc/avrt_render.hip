@@ -1481,8 +1481,13 @@ int avrt_autotune(avrt_ctx *c, const AvrtConfig *cfg, int mode, AvrtTuneReport *
         return AVRT_OK;
     };
     for (int v = 0; v < kNumTraceVariants; v++) {
-        AvrtTuning t{kTraceVariants[v][0], kTraceVariants[v][1], 0u, 0u};
+        AvrtTuning t{kTraceVariants[v][0], kTraceVariants[v][1], 0u, 0u, cur.graph};
         if ((rc = consider(t))) break;
+    }
+    if (rc == AVRT_OK) {
+        AvrtTuning t = best;
+        t.graph = t.graph ? 0u : 1u;
+        rc = consider(t);
     }
     for (uint32_t bits : {12u, 24u}) {
         if (rc) break;

@@ -356,9 +356,9 @@ int avrt_get_tuning(const avrt_ctx *ctx, AvrtTuning *out);
 /* AVRT_E_INVALID_ARG for values outside the lists above. */
 int avrt_set_tuning(avrt_ctx *ctx, const AvrtTuning *tuning);
 /* Autotuner: renders `config` (one warm-up and one timed frame per setting)
- * with each traversal variant, then the best with sort keys of 12 and 24
- * bits, then with 1/2 and 1/4 of the sample frames per chunk, keeps the
- * fastest and reports all trials. Needs an uploaded scene; overwrites the
+ * with each traversal variant, then the best with the bounce loop captured
+ * and not, then with sort keys of 12 and 24 bits, then with 1/2 and 1/4 of
+ * the sample frames per chunk, keeps the fastest and reports all trials. Needs an uploaded scene; overwrites the
  * accumulation image (render again afterwards). */
 int avrt_autotune(avrt_ctx *ctx, const AvrtConfig *config, int mode, AvrtTuneReport *report);
 

@@ -250,7 +250,10 @@ def test_autotune_keeps_the_fastest(c3):
         ref = r.ReadImage()
         rep = r.Autotune(cfg, abi.MODE_RAY_TRACE)
         say(f"autotune: best {rep['best']}, {len(rep['trials'])} trials")
-        assert len(rep["trials"]) == 6 + 2 + 2
+        assert len(rep["trials"]) == 6 + 1 + 2 + 2
+        # the variants keep the context's graph setting; one trial flips it
+        assert [t["graph"] for t, _ in rep["trials"][:6]] == [1] * 6
+        assert rep["trials"][6][0]["graph"] == 0
         best_ms = min(ms for _, ms in rep["trials"])
         assert (rep["best"], best_ms) in rep["trials"]
         assert r.GetTuning() == rep["best"]
