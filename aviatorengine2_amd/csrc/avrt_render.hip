@@ -644,7 +644,16 @@ int avrt_upload_textures(avrt_ctx *c, const uint8_t *rgba8_srgb, uint32_t width,
     if (!rgba8_srgb || width == 0 || height == 0) return set_err(c, AVRT_E_INVALID_ARG, "bad texture arguments");
     if (width > 8192 || height > 8192) return set_err(c, AVRT_E_CAPACITY, "texture layers are limited to 8192^2");
     HIPX(c, hipSetDevice(c->device));
-    int rc = upload_vec(c, c->d_tex, rgba8_srgb, static_cast<size_t>(width) * height * layers * 4);
+    // into the tiled layout the sampler reads (tex_texel)
+    const size_t per_layer = tex_layer_texels(width, height);
+    const uint32_t tx = tex_tiles_x(width);
+    std::vector<uint32_t> tiled(per_layer * layers, 0u);
+    for (uint32_t l = 0; l < layers; l++)
+        for (uint32_t j = 0; j < height; j++)
+            for (uint32_t i = 0; i < width; i++)
+                std::memcpy(&tiled[l * per_layer + tex_texel(i, j, tx)],
+                            rgba8_srgb + ((static_cast<size_t>(l) * height + j) * width + i) * 4, 4);
+    int rc = upload_vec(c, c->d_tex, tiled.data(), tiled.size() * 4);
     if (rc) return rc;
     c->tw = width;
     c->th = height;

@@ -92,11 +92,12 @@ __device__ F3 texture_color(const SceneDev &sc, const FrameParams &fp, uint32_t 
     const int i0 = wrapi(static_cast<int>(fmodf(x0, static_cast<float>(W))), W);
     const int j0 = wrapi(static_cast<int>(fmodf(y0, static_cast<float>(H))), H);
     const int i1 = (i0 + 1) % W, j1 = (j0 + 1) % H;
-    const uchar4 *base = sc.tex + static_cast<size_t>(layer) * W * H;
-    const uchar4 a = base[static_cast<size_t>(j0) * W + i0];
-    const uchar4 b = base[static_cast<size_t>(j0) * W + i1];
-    const uchar4 c = base[static_cast<size_t>(j1) * W + i0];
-    const uchar4 e = base[static_cast<size_t>(j1) * W + i1];
+    const uchar4 *base = sc.tex + static_cast<size_t>(layer) * tex_layer_texels(fp.tex_w, fp.tex_h);
+    const uint32_t tx = tex_tiles_x(fp.tex_w);
+    const uchar4 a = base[tex_texel(i0, j0, tx)];
+    const uchar4 b = base[tex_texel(i1, j0, tx)];
+    const uchar4 c = base[tex_texel(i0, j1, tx)];
+    const uchar4 e = base[tex_texel(i1, j1, tx)];
     const F3 t00 = f3(lut[a.x], lut[a.y], lut[a.z]);
     const F3 t10 = f3(lut[b.x], lut[b.y], lut[b.z]);
     const F3 t01 = f3(lut[c.x], lut[c.y], lut[c.z]);
