@@ -699,10 +699,12 @@ struct Traversal {
                 const float qx = -g.b[0], qy = -g.b[1], qz = -g.b[2], qw = g.b[3];
                 ao = quat_rotate(qx, qy, qz, qw, o - f3(g.a[0], g.a[1], g.a[2]));
                 ad = quat_rotate(qx, qy, qz, qw, d);
-                const float s = g.a[3];
-                if (s != 1.0f) {
-                    ao = ao / s;
-                    ad = ad / s;
+                // uniform-scale extension: times 1/scale, the reciprocal
+                // packed with the instance's record
+                const float inv_s = g.a[3];
+                if (inv_s != 1.0f) {
+                    ao = ao * inv_s;
+                    ad = ad * inv_s;
                 }
                 ainv = f3(1.0f / ad.x, 1.0f / ad.y, 1.0f / ad.z);
                 cur_obj = slot;

@@ -96,7 +96,7 @@ static_assert(sizeof(ObjGeom) == 64, "ObjGeom is 64 bytes");
 // Sphere:   a = (P.xyz, radius)  b = (P2 - P, 0)
 // Quad/Tri: a = (P.xyz, dot(n,P)) b = (u.xyz, 0) c = (v.xyz, 0) d.xyz = w as bits
 // Box:      a = (P.xyz, sin yaw)  b = (half.xyz, cos yaw)
-// Mesh:     a = (pos.xyz, scale)  b = (qx,qy,qz,qw)  c = (blas super-root,
+// Mesh:     a = (pos.xyz, 1/scale)  b = (qx,qy,qz,qw)  c = (blas super-root,
 //           triangle base, vertex base, index base) as bits
 
 // Texture layers are stored in tiles of 8 x 4 texels (128 bytes, one cache

@@ -655,7 +655,7 @@ __device__ ObjGeom make_geom(const AvrtObject &o, const AvrtMeshInstance *inst, 
         if (!(nl < 1e-8f)) flags |= kGeomValid;
     } else if (o.shapeType == AVRT_SHAPE_MESH) {
         const AvrtMeshInstance m = inst[rank];
-        g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = (m.scale == 0.0f) ? 1.0f : m.scale;
+        g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = 1.0f / ((m.scale == 0.0f) ? 1.0f : m.scale);
         g.b[0] = m.qx; g.b[1] = m.qy; g.b[2] = m.qz; g.b[3] = m.qw;
         const uint4 c = geomc[rank];
         g.c[0] = __uint_as_float(c.x); g.c[1] = __uint_as_float(c.y);

@@ -276,7 +276,7 @@ int build_geom_records(const AvrtObject *objects, uint32_t n, const AvrtMeshInst
                 return AVRT_E_INVALID_ARG;
             }
             const AvrtMeshInstance &m = inst[o.instanceIndex];
-            g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = (m.scale == 0.0f) ? 1.0f : m.scale;
+            g.a[0] = m.px; g.a[1] = m.py; g.a[2] = m.pz; g.a[3] = 1.0f / ((m.scale == 0.0f) ? 1.0f : m.scale);
             g.b[0] = m.qx; g.b[1] = m.qy; g.b[2] = m.qz; g.b[3] = m.qw;
             if (m.triangleCount > 0) {
                 auto it = blas_root.find(m.blasBase);

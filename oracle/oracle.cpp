@@ -430,13 +430,15 @@ static bool hit_mesh(const Scene &s, const Object &o, uint32_t id, const Ray &r,
 {
     const Inst &in = s.inst[o.instanceIndex];
     if (cnt) cnt->insts++;
-    const float scale = (in.scale == 0.0f) ? 1.0f : in.scale;
+    // uniform-scale extension: the rotated ray times 1/scale (the reciprocal
+    // once per instance)
+    const float inv_scale = 1.0f / ((in.scale == 0.0f) ? 1.0f : in.scale);
     Ray lr;
     lr.o = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.o - v3(in.px, in.py, in.pz));
     lr.d = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.d);
-    if (scale != 1.0f) {
-        lr.o = lr.o / scale;
-        lr.d = lr.d / scale;
+    if (inv_scale != 1.0f) {
+        lr.o = lr.o * inv_scale;
+        lr.d = lr.d * inv_scale;
     }
     lr.time = r.time;
     const V3 inv = v3(1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z);
@@ -1000,13 +1002,13 @@ static bool occluded(const Scene &s, const Ray &r, float near, float far, Counte
                 if (cnt) cnt->objs++;
                 if (o.shapeType == MESH) {
                     const Inst &in = s.inst[o.instanceIndex];
-                    const float scale = (in.scale == 0.0f) ? 1.0f : in.scale;
+                    const float inv_scale = 1.0f / ((in.scale == 0.0f) ? 1.0f : in.scale);
                     Ray lr;
                     lr.o = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.o - v3(in.px, in.py, in.pz));
                     lr.d = quat_rotate(-in.qx, -in.qy, -in.qz, in.qw, r.d);
-                    if (scale != 1.0f) {
-                        lr.o = lr.o / scale;
-                        lr.d = lr.d / scale;
+                    if (inv_scale != 1.0f) {
+                        lr.o = lr.o * inv_scale;
+                        lr.d = lr.d * inv_scale;
                     }
                     const V3 li = v3(1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z);
                     int st2[64];
