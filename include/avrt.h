@@ -40,7 +40,7 @@ extern "C" {
 #define AVRT_E_DEVICE (-4)        /* a HIP call failed                             */
 #define AVRT_E_STATE (-5)         /* call order violated (e.g. render before load) */
 #define AVRT_E_BUDGET (-6)        /* serving: job does not fit the HBM budget      */
-#define AVRT_E_UNSUPPORTED (-7)   /* feature outside this build                    */
+#define AVRT_E_UNSUPPORTED (-7)   /* feature this library does not implement       */
 
 /* Material and shape enums, values as uploaded by the reference.
  * MaterialType: scene/material.h:11-18. BodyShape: scene/gpu_types.h:14-21. */
