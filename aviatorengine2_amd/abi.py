@@ -164,6 +164,7 @@ _SIG = {
     "avrt_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(AvrtTuning)]),
     "avrt_autotune": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(AvrtTuneReport)]),
     "avrt_build_bvh": (ctypes.c_int, [_P, _U32, _P, ctypes.POINTER(_U32), _P]),
+    "avrt_rebuild_tlas": (ctypes.c_int, [_P, _P, _U32, _P, ctypes.POINTER(_U32), _P]),
     "avrt_build_blas": (ctypes.c_int, [_P, _U32, _P, _U32, _P, ctypes.POINTER(_U32), _P, _P]),
     "avrt_instance_bounds": (ctypes.c_int, [_P, _U32, _P, _P]),
     "avrt_scene_set_authoring": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _P]),

@@ -50,6 +50,11 @@ struct SceneRebuild {
     float last_ms = 0.0f;
 };
 
+// Buffers of avrt_rebuild_tlas (the binned-SAH build over caller AABBs).
+struct TlasBuild {
+    DevBuf bounds, nodes, depth, range, order, scratch, info;
+};
+
 // Targets and queues of the hybrid frame (avrt_render_hybrid): the ray-cast
 // G-buffer at full resolution, the two history slots (accumulation image and
 // per-pixel geometry), the reduced-grid lighting and the per-cell state of
@@ -150,6 +155,7 @@ struct avrt_ctx {
     hipEvent_t gev_io[4] = {};
 
     avrt::SceneRebuild rb;
+    avrt::TlasBuild tb;
     avrt::HybridState hy;
     bool last_hybrid = false;    // the last render was avrt_render_hybrid
 

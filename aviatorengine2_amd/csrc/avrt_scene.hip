@@ -1211,6 +1211,45 @@ int avrt_scene_read(avrt_ctx *c, AvrtBVHNode *nodes, uint32_t *node_count, AvrtO
     return AVRT_OK;
 }
 
+int avrt_rebuild_tlas(avrt_ctx *c, const AvrtAABB *bounds, uint32_t n, AvrtBVHNode *out_nodes,
+                      uint32_t *out_node_count, uint32_t *out_order)
+{
+    if (c == nullptr || (n && bounds == nullptr)) return AVRT_E_INVALID_ARG;
+    if (n == 0) {
+        if (out_node_count) *out_node_count = 0;
+        return AVRT_OK;
+    }
+    HIPX(c, hipSetDevice(c->device));
+    TlasBuild &t = c->tb;
+    size_t sbytes = 0;
+    carve(nullptr, n, &sbytes);
+    int rc = 0;
+    if ((rc = upload_vec(c, t.bounds, bounds, sizeof(AvrtAABB) * n)) ||
+        (rc = ensure(c, t.nodes, sizeof(AvrtBVHNode) * 2 * static_cast<size_t>(n))) ||
+        (rc = ensure(c, t.depth, sizeof(uint32_t) * 2 * static_cast<size_t>(n))) ||
+        (rc = ensure(c, t.range, sizeof(uint2) * 2 * static_cast<size_t>(n))) ||
+        (rc = ensure(c, t.order, sizeof(uint32_t) * static_cast<size_t>(n))) || (rc = ensure(c, t.scratch, sbytes)) ||
+        (rc = ensure(c, t.info, 32)))
+        return rc;
+    HIPX(c, hipMemsetAsync(t.info.p, 0, 32, c->stream));
+    BuildWork w = carve(t.scratch.p, n, &sbytes);
+    hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kBT), 0, c->stream, n, t.bounds.as<AvrtAABB>(), w,
+                       t.nodes.as<AvrtBVHNode>(), t.depth.as<uint32_t>(), t.range.as<uint2>(), t.order.as<uint32_t>(),
+                       t.info.as<uint32_t>());
+    HIPX(c, hipGetLastError());
+    uint32_t info[8];
+    HIPX(c, hipMemcpyAsync(info, t.info.p, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    if (info[1] & kErrBuild) return set_err(c, AVRT_E_DEVICE, "device TLAS build produced an inconsistent partition");
+    if (info[0] == 0 || info[0] > 2 * n) return set_err(c, AVRT_E_DEVICE, "device TLAS build returned no tree");
+    if (out_node_count) *out_node_count = info[0];
+    if (out_nodes)
+        HIPX(c, hipMemcpyAsync(out_nodes, t.nodes.p, sizeof(AvrtBVHNode) * info[0], hipMemcpyDeviceToHost, c->stream));
+    if (out_order) HIPX(c, hipMemcpyAsync(out_order, t.order.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipStreamSynchronize(c->stream));
+    return AVRT_OK;
+}
+
 int avrt_update_vertices(avrt_ctx *c, const AvrtVertex *vertices, uint32_t first, uint32_t count)
 {
     if (c == nullptr || (count && vertices == nullptr)) return AVRT_E_INVALID_ARG;

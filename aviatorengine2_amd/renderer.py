@@ -206,6 +206,18 @@ class Renderer:
         return out
 
     # -- device-side scene rebuild (RebuildAcceleration on the GPU) ----------
+    def RebuildTLAS(self, bounds):
+        """BuildBVH over `bounds` (AABB_DTYPE) on the device: (nodes, order),
+        the same arrays as the host builder."""
+        b = np.ascontiguousarray(bounds, abi.AABB_DTYPE)
+        n = len(b)
+        nodes = np.zeros(max(2 * n, 1), abi.NODE_DTYPE)
+        order = np.zeros(max(n, 1), np.uint32)
+        cnt = ctypes.c_uint32(0)
+        self._check(self.lib.avrt_rebuild_tlas(self.ctx, ptr(b), n, ptr(nodes), ctypes.byref(cnt), ptr(order)),
+                    "avrt_rebuild_tlas")
+        return nodes[:cnt.value].copy(), order[:n].copy()
+
     def SetAuthoring(self, objects, instances, local_bounds):
         """Objects in authoring (GatherRenderables) order, one instance record
         per mesh object in the same order, and each instance's mesh-local

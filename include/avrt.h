@@ -374,6 +374,14 @@ int avrt_build_bvh(const AvrtAABB *bounds, uint32_t n,
                    AvrtBVHNode *out_nodes, uint32_t *out_node_count,
                    uint32_t *out_order);
 
+/* BuildBVH (scene/bvh.h:275-295) on the device over caller AABBs: the
+ * per-frame TLAS build that RebuildAcceleration runs on the CPU
+ * (renderer.cpp:1087-1130), as one 1024-thread workgroup. Same outputs and
+ * sizes as avrt_build_bvh, bit for bit. Does not touch the context's scene. */
+int avrt_rebuild_tlas(avrt_ctx *ctx, const AvrtAABB *bounds, uint32_t n,
+                      AvrtBVHNode *out_nodes, uint32_t *out_node_count,
+                      uint32_t *out_order);
+
 /* MeshLibrary::Add (scene/mesh_library.h:91-159) for one mesh: triangle
  * AABBs, BuildBVH, and the index buffer reordered by the BLAS permutation.
  * positions are xyz per vertex; tri_indices are 3 per triangle. out_nodes must

@@ -249,3 +249,26 @@ def test_graph_path_trace_accumulates_like_stages():
         outs.append(imgs)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 100, 1000, 4097, 20000])
+def test_rebuild_tlas_over_aabbs_matches_host_builder(n):
+    """avrt_rebuild_tlas (BuildBVH as one device workgroup) over random boxes,
+    with repeated boxes, flat boxes and a cluster of equal centroids, gives
+    the host builder's nodes and permutation byte for byte."""
+    rng = np.random.default_rng(1000 + n)
+    lo = rng.uniform(-100, 100, (n, 3)).astype(np.float32)
+    ext = rng.uniform(0, 5, (n, 3)).astype(np.float32)
+    ext[rng.random(n) < 0.1, rng.integers(0, 3)] = 0.0          # flat along one axis
+    if n > 8:
+        lo[: n // 8] = lo[0]                                      # repeated boxes
+        ext[: n // 8] = ext[0]
+    b = np.zeros(n, abi.AABB_DTYPE)
+    b["min_x"], b["min_y"], b["min_z"] = lo[:, 0], lo[:, 1], lo[:, 2]
+    b["max_x"], b["max_y"], b["max_z"] = lo[:, 0] + ext[:, 0], lo[:, 1] + ext[:, 1], lo[:, 2] + ext[:, 2]
+    h_nodes, h_order = scenes.Builder().build_bvh(b)
+    print(f"[gpu] device TLAS build over {n} boxes", flush=True)
+    with Renderer().Initialize() as r:
+        d_nodes, d_order = r.RebuildTLAS(b)
+    assert d_nodes.tobytes() == h_nodes.tobytes()
+    assert np.array_equal(d_order, h_order)
