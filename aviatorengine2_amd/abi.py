@@ -151,6 +151,8 @@ _SIG = {
     "avrt_upload_scene": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _P, _U32, _P, _U32]),
     "avrt_upload_textures": (ctypes.c_int, [_P, _P, _U32, _U32, _U32]),
     "avrt_render": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+    "avrt_render_async": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+    "avrt_wait": (ctypes.c_int, [_P]),
     "avrt_render_hybrid": (ctypes.c_int, [_P, _P]),
     "avrt_halton": (ctypes.c_float, [_U32, _U32]),
     "avrt_read_gbuffer": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_size_t]),

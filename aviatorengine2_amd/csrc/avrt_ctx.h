@@ -50,6 +50,15 @@ struct SceneRebuild {
     float last_ms = 0.0f;
 };
 
+// Pinned host staging of one chunk of the captured bounce loop: the frame
+// constants its replay copies in, and the counters it copies back.
+struct GraphSlot {
+    void *frame = nullptr;
+    void *counters = nullptr;
+    uint32_t n = 0, depth = 0;
+    bool wave_nee = false;
+};
+
 // Buffers of avrt_rebuild_tlas (the binned-SAH build over caller AABBs).
 struct TlasBuild {
     DevBuf bounds, nodes, depth, range, order, scratch, info;
@@ -150,7 +159,10 @@ struct avrt_ctx {
     bool use_graph = true;
     std::vector<std::pair<std::vector<unsigned char>, hipGraphExec_t>> graphs; // most recent last, at most 4
     avrt::DevBuf d_gframe, gcounters;
-    void *h_gframe = nullptr;
+    std::vector<avrt::GraphSlot> g_slots;   // pinned per-chunk staging, grown on demand
+    // avrt_render_async: a frame is queued and its counts not yet read
+    bool pending = false;
+    int pending_chunks = 0;
     hipEvent_t gev[kGraphMaxDepth][5] = {};
     hipEvent_t gev_io[4] = {};
 
@@ -181,6 +193,8 @@ dev::SceneDev scene_dev(const avrt_ctx *c);
 void camera_params(const AvrtConfig &cfg, FrameParams &fp);
 int frame_params(avrt_ctx *c, const AvrtConfig *cfg, FrameParams &fp);
 int validate_config(avrt_ctx *c, const AvrtConfig *cfg);
+// Waits for a frame queued by avrt_render_async and reads its counts.
+int finish_pending(avrt_ctx *c);
 int ensure_image(avrt_ctx *c, uint32_t w, uint32_t h);
 float elapsed(hipEvent_t a, hipEvent_t b);
 // Hybrid frame unit: sRGB table upload (once per context) and buffer release.

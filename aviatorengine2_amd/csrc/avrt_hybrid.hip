@@ -918,7 +918,9 @@ float avrt_halton(uint32_t index, uint32_t base)
 int avrt_render_hybrid(avrt_ctx *c, const AvrtConfig *cfg)
 {
     if (c == nullptr) return AVRT_E_INVALID_ARG;
-    int rc = validate_config(c, cfg);
+    int rc = finish_pending(c);
+    if (rc) return rc;
+    rc = validate_config(c, cfg);
     if (rc) return rc;
     if (c->has_volume) return set_err(c, AVRT_E_UNSUPPORTED, "ISOTROPIC (volume) objects are not supported");
     HIPX(c, hipSetDevice(c->device));

@@ -505,6 +505,7 @@ int avrt_create(const AvrtCreateOptions *opts, avrt_ctx **out_ctx)
 int avrt_destroy(avrt_ctx *c)
 {
     if (c == nullptr) return AVRT_E_INVALID_ARG;
+    (void)finish_pending(c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
@@ -522,7 +523,10 @@ int avrt_destroy(avrt_ctx *c)
     drop_graph(c);
     release(c->d_gframe);
     release(c->gcounters);
-    if (c->h_gframe) (void)hipHostFree(c->h_gframe);
+    for (avrt::GraphSlot &gs : c->g_slots) {
+        if (gs.frame) (void)hipHostFree(gs.frame);
+        if (gs.counters) (void)hipHostFree(gs.counters);
+    }
     for (auto &row : c->gev)
         for (hipEvent_t e : row)
             if (e) (void)hipEventDestroy(e);
@@ -1187,14 +1191,16 @@ static void drop_graph(avrt_ctx *c)
     c->graphs.clear();
 }
 
+// Records (once per launch shape) and replays the bounce loop of one chunk,
+// then queues the read-back of its counters into pinned slot `slot`; no host
+// wait.
 template <int MODE>
-static int run_chunk_graph(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns)
+static int enqueue_chunk_graph(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns, uint32_t slot)
 {
     int rc;
     if ((rc = ensure(c, c->d_gframe, sizeof(GraphFrame))) ||
         (rc = ensure(c, c->gcounters, static_cast<size_t>(kGraphMaxDepth) * kGraphSlotWords * sizeof(uint32_t))))
         return rc;
-    if (c->h_gframe == nullptr) HIPX(c, hipHostMalloc(&c->h_gframe, sizeof(GraphFrame), hipHostMallocDefault));
     for (auto &row : c->gev)
         for (hipEvent_t &e : row)
             if (e == nullptr) HIPX(c, hipEventCreate(&e));
@@ -1261,34 +1267,62 @@ static int run_chunk_graph(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint
         c->graphs.emplace_back(key, exec);
     }
     const uint32_t n = fp.width * fp.height * ns;
+    // per-chunk pinned slots: the constants this chunk's copy reads and the
+    // counters its read-back writes, so that several chunks can be in flight
+    if (slot >= c->g_slots.size()) c->g_slots.resize(slot + 1);
+    GraphSlot &gs = c->g_slots[slot];
+    const size_t cnt_words = static_cast<size_t>(kGraphMaxDepth) * kGraphSlotWords;
+    if (gs.frame == nullptr) HIPX(c, hipHostMalloc(&gs.frame, sizeof(GraphFrame), hipHostMallocDefault));
+    if (gs.counters == nullptr) HIPX(c, hipHostMalloc(&gs.counters, cnt_words * sizeof(uint32_t), hipHostMallocDefault));
     GraphFrame h;
     std::memset(&h, 0, sizeof(h));
     h.fp = fp;
     h.s0 = s0;
     h.n0 = n;
-    std::memcpy(c->h_gframe, &h, sizeof(h));
-    HIPX(c, hipMemcpyAsync(g.dg, c->h_gframe, sizeof(GraphFrame), hipMemcpyHostToDevice, c->stream));
+    std::memcpy(gs.frame, &h, sizeof(h));
+    gs.n = n;
+    gs.depth = fp.depth;
+    gs.wave_nee = wave_nee;
+    HIPX(c, hipMemcpyAsync(g.dg, gs.frame, sizeof(GraphFrame), hipMemcpyHostToDevice, c->stream));
     HIPX(c, hipGraphLaunch(exec, c->stream));
-    std::vector<uint32_t> hc(static_cast<size_t>(fp.depth) * kGraphSlotWords);
-    HIPX(c, hipMemcpyAsync(hc.data(), g.gc, hc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPX(c, hipStreamSynchronize(c->stream));
+    HIPX(c, hipMemcpyAsync(gs.counters, g.gc, static_cast<size_t>(fp.depth) * kGraphSlotWords * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    return AVRT_OK;
+}
 
-    c->stats.paths += n;
-    c->stats.rays_primary += n;
-    c->stats.ms_other += elapsed(c->gev_io[0], c->gev_io[1]) + elapsed(c->gev_io[2], c->gev_io[3]);
-    uint32_t nb = n;
-    for (uint32_t b = 0; b < fp.depth && nb > 0; b++) {
-        const uint32_t *w = hc.data() + static_cast<size_t>(b) * kGraphSlotWords;
+// Ray counts (and, with `timing`, stage times from the graph's event nodes)
+// of a finished chunk into the context's stats.
+static void finish_chunk_graph(avrt_ctx *c, uint32_t slot, bool timing)
+{
+    const GraphSlot &gs = c->g_slots[slot];
+    const uint32_t *hc = static_cast<const uint32_t *>(gs.counters);
+    c->stats.paths += gs.n;
+    c->stats.rays_primary += gs.n;
+    if (timing) c->stats.ms_other += elapsed(c->gev_io[0], c->gev_io[1]) + elapsed(c->gev_io[2], c->gev_io[3]);
+    uint32_t nb = gs.n;
+    for (uint32_t b = 0; b < gs.depth && nb > 0; b++) {
+        const uint32_t *w = hc + static_cast<size_t>(b) * kGraphSlotWords;
         if (b > 0) c->stats.rays_bounce += nb;
         c->stats.launches_extend++;
-        c->stats.ms_extend += elapsed(c->gev[b][0], c->gev[b][1]);
-        c->stats.ms_shade += elapsed(c->gev[b][1], c->gev[b][2]);
-        if (wave_nee) {
+        if (timing) {
+            c->stats.ms_extend += elapsed(c->gev[b][0], c->gev[b][1]);
+            c->stats.ms_shade += elapsed(c->gev[b][1], c->gev[b][2]);
+        }
+        if (gs.wave_nee) {
             c->stats.rays_shadow += w[2 * kCtr] + w[3 * kCtr];
-            c->stats.ms_shadow += elapsed(c->gev[b][3], c->gev[b][4]);
+            if (timing) c->stats.ms_shadow += elapsed(c->gev[b][3], c->gev[b][4]);
         }
         nb = w[0];
     }
+}
+
+template <int MODE>
+static int run_chunk_graph(avrt_ctx *c, const FrameParams &fp, uint32_t s0, uint32_t ns)
+{
+    int rc = enqueue_chunk_graph<MODE>(c, fp, s0, ns, 0);
+    if (rc) return rc;
+    HIPX(c, hipStreamSynchronize(c->stream));
+    finish_chunk_graph(c, 0, true);
     return AVRT_OK;
 }
 
@@ -1308,12 +1342,23 @@ int avrt::frame_params(avrt_ctx *c, const AvrtConfig *cfg, FrameParams &fp)
     return AVRT_OK;
 }
 
-extern "C" {
-
-int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
+// Counts of a frame queued by avrt_render_async, once its last chunk is done.
+int avrt::finish_pending(avrt_ctx *c)
 {
-    if (c == nullptr) return AVRT_E_INVALID_ARG;
-    int rc = validate_config(c, cfg);
+    if (!c->pending) return AVRT_OK;
+    c->pending = false;
+    HIPX(c, hipSetDevice(c->device));
+    HIPX(c, hipEventSynchronize(c->ev1));
+    for (int k = 0; k < c->pending_chunks; k++) finish_chunk_graph(c, static_cast<uint32_t>(k), false);
+    c->stats.ms_total = elapsed(c->ev0, c->ev1);
+    return AVRT_OK;
+}
+
+static int render_frame(avrt_ctx *c, const AvrtConfig *cfg, int mode, bool async)
+{
+    int rc = finish_pending(c);
+    if (rc) return rc;
+    rc = validate_config(c, cfg);
     if (rc) return rc;
     if (mode != AVRT_MODE_PATH_TRACE && mode != AVRT_MODE_RAY_TRACE) return set_err(c, AVRT_E_INVALID_ARG, "unknown mode");
     if (mode == AVRT_MODE_PATH_TRACE && c->has_tri_or_mesh)
@@ -1342,9 +1387,24 @@ int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
     if (c->chunk_spp) per_chunk = std::min(per_chunk, c->chunk_spp);
     rc = ensure_paths(c, static_cast<size_t>(P) * per_chunk, c->nlights);
     if (rc) return rc;
+    const bool graph = graph_eligible(c, fp, mode);
+    if (async && graph) {
+        // every chunk queued behind the previous one; counts read at avrt_wait
+        uint32_t k = 0;
+        for (uint32_t s0 = 0; s0 < S; s0 += per_chunk, k++) {
+            const uint32_t ns = std::min(per_chunk, S - s0);
+            rc = (mode == AVRT_MODE_PATH_TRACE) ? enqueue_chunk_graph<0>(c, fp, s0, ns, k)
+                                                : enqueue_chunk_graph<1>(c, fp, s0, ns, k);
+            if (rc) return rc;
+        }
+        HIPX(c, hipEventRecord(c->ev1, c->stream));
+        c->pending = true;
+        c->pending_chunks = static_cast<int>(k);
+        return AVRT_OK;
+    }
     for (uint32_t s0 = 0; s0 < S; s0 += per_chunk) {
         const uint32_t ns = std::min(per_chunk, S - s0);
-        if (graph_eligible(c, fp, mode))
+        if (graph)
             rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk_graph<0>(c, fp, s0, ns) : run_chunk_graph<1>(c, fp, s0, ns);
         else
             rc = (mode == AVRT_MODE_PATH_TRACE) ? run_chunk<0>(c, fp, s0, ns) : run_chunk<1>(c, fp, s0, ns);
@@ -1354,6 +1414,26 @@ int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
     HIPX(c, hipEventSynchronize(c->ev1));
     c->stats.ms_total = elapsed(c->ev0, c->ev1);
     return AVRT_OK;
+}
+
+extern "C" {
+
+int avrt_render(avrt_ctx *c, const AvrtConfig *cfg, int mode)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    return render_frame(c, cfg, mode, false);
+}
+
+int avrt_render_async(avrt_ctx *c, const AvrtConfig *cfg, int mode)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    return render_frame(c, cfg, mode, true);
+}
+
+int avrt_wait(avrt_ctx *c)
+{
+    if (c == nullptr) return AVRT_E_INVALID_ARG;
+    return finish_pending(c);
 }
 
 int avrt_get_stats(const avrt_ctx *c, AvrtStats *out)

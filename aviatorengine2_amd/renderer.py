@@ -102,6 +102,18 @@ class Renderer:
         self._check(self.lib.avrt_render(self.ctx, c.ctypes.data_as(ctypes.c_void_p), int(mode)), "avrt_render")
         return self.Stats()
 
+    def RenderFrameAsync(self, config, mode):
+        """Queues the frame and returns at once; Wait() blocks until it is done
+        and returns its stats."""
+        c = np.array(config, dtype=abi.CONFIG_DTYPE, copy=True).reshape(1)
+        self.width, self.height = int(c["width"][0]), int(c["height"][0])
+        self._check(self.lib.avrt_render_async(self.ctx, c.ctypes.data_as(ctypes.c_void_p), int(mode)),
+                    "avrt_render_async")
+
+    def Wait(self):
+        self._check(self.lib.avrt_wait(self.ctx), "avrt_wait")
+        return self.Stats()
+
     def RenderHybrid(self, config):
         """One hybrid frame (avrt_render_hybrid): the config's hybrid uniforms
         as Renderer::RenderFrame sets them (see hybrid.HybridFrames)."""

@@ -269,6 +269,20 @@ int avrt_upload_textures(avrt_ctx *ctx, const uint8_t *rgba8_srgb,
  * is finished. */
 int avrt_render(avrt_ctx *ctx, const AvrtConfig *config, int mode);
 
+/* The same frame as avrt_render, queued on the context's stream; returns
+ * without waiting, so the caller's thread can prepare the next frame. No
+ * counterpart in the reference (its one command buffer per frame is
+ * submitted and not waited on, renderer.cpp:1426-1588). avrt_wait blocks
+ * until the frame is done and makes its counts available to
+ * avrt_get_stats; the stage times of an asynchronous frame are not split
+ * out (ms_total only). The next avrt_render, avrt_render_async,
+ * avrt_render_hybrid or avrt_destroy waits first; any other call on the
+ * context is ordered after the frame on its stream. A frame that needs host
+ * reads between its stages (graph off, sort_bits >= 3, depth > 16, more than
+ * four lights) renders synchronously. */
+int avrt_render_async(avrt_ctx *ctx, const AvrtConfig *config, int mode);
+int avrt_wait(avrt_ctx *ctx);
+
 /* Renderer::RenderFrame in hybrid mode (renderer.cpp:1330-1583): the
  * G-buffer pass, deferred.comp over the trace grid and, below full
  * resolution, upsample.comp. The G-buffer is ray-cast, not rasterized: for

@@ -272,3 +272,39 @@ def test_rebuild_tlas_over_aabbs_matches_host_builder(n):
         d_nodes, d_order = r.RebuildTLAS(b)
     assert d_nodes.tobytes() == h_nodes.tobytes()
     assert np.array_equal(d_order, h_order)
+
+
+@pytest.mark.parametrize("chunk", [0, 3])
+def test_async_frames_equal_sync_frames(chunk):
+    """avrt_render_async + avrt_wait give the synchronous frames' images and
+    ray counts, frame after frame with scene steps in between; a read queued
+    right after the asynchronous call (before the wait) sees the finished
+    frame, being ordered after it on the context's stream."""
+    spec = scenes.instanced(n_instances=256, tex_size=256, width=320, height=180, spp=4, depth=8)
+    builder = scenes.Builder()
+    library = scenes.mesh_library(spec.meshes, builder)
+    packed = scenes.pack(spec, builder, library)
+    objs, inst, local = scenes.authoring(spec, library)
+    outs = []
+    for use_async in (True, False):
+        with Renderer().Initialize() as r:
+            r.SetTuning(samples_per_chunk=chunk)
+            r.UploadMeshGeometry(library[0], library[1], library[2])
+            r.UploadTextures(spec.texture)
+            r.SetAuthoring(objs, inst, local)
+            frames = []
+            for f in range(3):
+                print(f"[gpu] async {use_async} chunk {chunk} frame {f}", flush=True)
+                r.SceneStep(spec.velocities, 1.0 / 60.0)
+                if use_async:
+                    r.RenderFrameAsync(packed.config, abi.MODE_RAY_TRACE)
+                    img = r.ReadImage()
+                    st = r.Wait()
+                else:
+                    st = r.RenderFrame(packed.config, abi.MODE_RAY_TRACE)
+                    img = r.ReadImage()
+                frames.append((img, st["rays_primary"], st["rays_bounce"], st["rays_shadow"], st["paths"]))
+        outs.append(frames)
+    for f, (a, b) in enumerate(zip(*outs)):
+        assert a[1:] == b[1:], (f, a[1:], b[1:])
+        assert np.array_equal(a[0], b[0]), f
