@@ -517,7 +517,8 @@ int avrt_destroy(avrt_ctx *c)
                       &c->rb.inst_geomc, &c->rb.mesh_rank, &c->rb.vel, &c->rb.bounds, &c->rb.nodes, &c->rb.order,
                       &c->rb.scratch, &c->rb.info, &c->rb.depth, &c->rb.range, &c->rb.lg_prefix, &c->rb.raster_prefix, &c->sro[0], &c->sro[1], &c->srd[0], &c->srd[1], &c->srl[0],
                       &c->srl[1], &c->nee_tp, &c->nee_contrib, &c->rs_key[0], &c->rs_key[1], &c->rs_val[0],
-                      &c->rs_val[1], &c->rs_tmp, &c->rs_o, &c->rs_d, &c->rs_t};
+                      &c->rs_val[1], &c->rs_tmp, &c->rs_o, &c->rs_d, &c->rs_t, &c->tb.bounds, &c->tb.nodes,
+                      &c->tb.depth, &c->tb.range, &c->tb.order, &c->tb.scratch, &c->tb.info};
     for (DevBuf *b : bufs) release(*b);
     hybrid_release(c);
     drop_graph(c);
