@@ -84,10 +84,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_raygen(SegLaunch L, const JobDev
                                                        float4 *Lpath)
 {
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
-    if (local >= L.count[j]) return;
+    const uint32_t stride = (L.blk[j + 1] - L.blk[j]) * kBlock, n = L.count[j];
     const JobDev &J = jobs[j];
-    raygen_path(J.fp, J.first_sample, J.ns, local, J.path_base + local, out, Lpath);
+    const FrameParams fp = J.fp;
+    for (uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x; local < n; local += stride)
+        raygen_path(fp, J.first_sample, J.ns, local, J.path_base + local, out, Lpath);
 }
 
 // One ray per lane (server option trace_schedule = 0): closest-hit queries
@@ -99,6 +100,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_trace(SegLaunch L, const JobDev 
                                                       const float4 *__restrict__ rlp, uint4 *__restrict__ hitA,
                                                       float *__restrict__ hitB)
 {
+    // one ray per lane and no loop: a wave ends with its slowest ray and the
+    // hardware starts the next in its place (a loop over rays would hold
+    // each wave's lanes to its slowest ray, ray after ray)
     __shared__ uint32_t lds_stack[kStackLds * kBlock];
     const uint32_t j = job_of(L, blockIdx.x);
     const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
@@ -154,8 +158,13 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
                                                       const float *__restrict__ hitB, PathBufs out, ShadowRays sr,
                                                       NeeBufs nee, float4 *Lpath, uint32_t *counters)
 {
+    // The job's workgroups stride over its queue (block-uniform trip count,
+    // so every lane of a wave takes part in the queue appends): the job
+    // lookup, the job's constants and the sRGB table are loaded once per
+    // workgroup instead of once per 256 paths.
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    const uint32_t nblk = L.blk[j + 1] - L.blk[j];
+    const uint32_t n = L.count[j];
     const SceneDev sc = jobs[j].sc;
     const FrameParams fp = jobs[j].fp;
     const uint32_t b = jobs[j].path_base;
@@ -163,12 +172,15 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
     const NeeBufs nj{nee.tp + b, nee.contrib + 4u * b};
     __shared__ float lut_mem[256];
     const float *lut = srgb_to_lds(lut_mem);
-    if (jobs[j].mode == AVRT_MODE_PATH_TRACE)
-        shade_path<0, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
-    else
-        shade_path<1, true>(sc, fp, local, L.count[j], bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
-                            at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
+    const bool pt = jobs[j].mode == AVRT_MODE_PATH_TRACE;
+    for (uint32_t k0 = (blockIdx.x - L.blk[j]) * kBlock; k0 < n; k0 += nblk * kBlock) {
+        if (pt)
+            shade_path<0, true>(sc, fp, k0 + threadIdx.x, n, bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
+                                at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
+        else
+            shade_path<1, true>(sc, fp, k0 + threadIdx.x, n, bounce, at(in, b), hitA + b, hitB + b, at(out, b), none,
+                                at(sr, 4u * b), nj, Lpath, counters + 4u * kCtr * j, lut);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
@@ -176,44 +188,40 @@ __global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, cons
                                                                ShadowRays out, float4 *contrib, uint32_t *counters)
 {
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
+    const uint32_t stride = (L.blk[j + 1] - L.blk[j]) * kBlock, n = L.count[j];
     const uint32_t s = 4u * jobs[j].path_base;
     const SceneDev sc = jobs[j].sc;
-    shadow_resolve_ray(sc, local, L.count[j], at(in, s), hitA + s, at(out, s), contrib + s,
-                       counters + 4u * kCtr * j + 3u * kCtr);
+    for (uint32_t k0 = (blockIdx.x - L.blk[j]) * kBlock; k0 < n; k0 += stride)
+        shadow_resolve_ray(sc, k0 + threadIdx.x, n, at(in, s), hitA + s, at(out, s), contrib + s,
+                           counters + 4u * kCtr * j + 3u * kCtr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_nee_finalize(SegLaunch L, const JobDev *__restrict__ jobs, NeeBufs nee,
                                                              float4 *Lpath)
 {
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t r = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
-    if (r >= L.count[j]) return;
+    const uint32_t stride = (L.blk[j + 1] - L.blk[j]) * kBlock, n = L.count[j];
     const JobDev &J = jobs[j];
+    const NeeBufs nj{nee.tp + J.path_base, nee.contrib + 4u * J.path_base};
     const uint32_t nl = J.fp.num_lights;
-    const float4 *contrib = nee.contrib + 4u * J.path_base;
-    F3 sum = f3(0.0f, 0.0f, 0.0f);
-    for (uint32_t i = 0; i < nl; i++) {
-        const float4 c = contrib[r * nl + i];
-        sum = sum + f3(c.x, c.y, c.z);
-    }
-    const float luma = fmaxf(sum.x, fmaxf(sum.y, sum.z));
-    if (luma > 8.0f) sum = sum * (8.0f / luma);
-    const float4 T = nee.tp[J.path_base + r];
-    add_radiance(Lpath, __float_as_uint(T.w), f3(T.x, T.y, T.z) * sum);
+    for (uint32_t r = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x; r < n; r += stride)
+        nee_finalize_record(r, nl, nj, Lpath);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
                                                         const float4 *Lpath)
 {
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t bq = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x; // band position of the pixel
-    if (bq >= L.count[j]) return;
+    const uint32_t stride = (L.blk[j + 1] - L.blk[j]) * kBlock, n = L.count[j];
     const JobDev &J = jobs[j];
-    if (J.mode == AVRT_MODE_PATH_TRACE)
-        resolve_pixel<0>(J.fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
-    else
-        resolve_pixel<1>(J.fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+    const FrameParams fp = J.fp;
+    // bq: band position of the pixel
+    for (uint32_t bq = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x; bq < n; bq += stride) {
+        if (J.mode == AVRT_MODE_PATH_TRACE)
+            resolve_pixel<0>(fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+        else
+            resolve_pixel<1>(fp, bq, J.first_sample, J.ns, Lpath + J.path_base, J.image, J.display);
+    }
 }
 
 } // namespace
@@ -244,6 +252,7 @@ struct avrt_server {
     AvrtServerOptions opts{};
     hipStream_t stream = nullptr;
     uint32_t resident[2] = {256, 256}; // workgroups of the persistent walk (extension, shadow) the device holds at once
+    uint32_t resident_shade = 2048;    // workgroups of a shading launch (each strides over its job's queue)
     std::string err;
     std::vector<ServeScene> scenes;
     std::unordered_map<uint64_t, ServeJob> jobs;
@@ -333,8 +342,9 @@ int ensure_round(avrt_server *s, size_t paths)
     return AVRT_OK;
 }
 
-// Launch table over `n` jobs with `count` entries each; returns the grid.
-uint32_t make_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
+// Launch table over `n` jobs with `count` entries each, one workgroup per
+// kBlock entries; returns the grid.
+uint32_t make_dense_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
 {
     L.njobs = n;
     uint32_t b = 0;
@@ -347,9 +357,11 @@ uint32_t make_launch(SegLaunch &L, const uint32_t *count, uint32_t n)
     return b;
 }
 
-// Launch table of the persistent walk: the resident workgroups, shared among
-// the jobs in proportion to their rays (at least one each, at most one per
-// kBlock rays); returns the grid. (Four times as many workgroups, so that
+// Launch table over `n` jobs with `count` entries each: `resident`
+// workgroups (what the device holds at once for the kernel) shared among the
+// jobs in proportion to their entries, at least one each and at most one per
+// kBlock entries; each workgroup strides over its job's entries. Returns the
+// grid. (For the persistent pass walk, four times as many workgroups, so that
 // those of cheap jobs' rays free room for the others, measured slower: 2.2-2.3
 // against 2.8 Grays/s on the mixed stream.)
 uint32_t make_persistent_launch(SegLaunch &L, const uint32_t *count, uint32_t n, uint32_t resident)
@@ -442,6 +454,9 @@ int avrt_server_create(const AvrtServerOptions *opts, avrt_server **out)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb1, k_seg_trace_ww<true>, kBlock, 0) != hipSuccess || nb1 <= 0) nb1 = 1;
     s->resident[0] = static_cast<uint32_t>(cu * nb0);
     s->resident[1] = static_cast<uint32_t>(cu * nb1);
+    int nbs = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbs, k_seg_shade, kBlock, 0) != hipSuccess || nbs <= 0) nbs = 1;
+    s->resident_shade = static_cast<uint32_t>(cu * nbs);
     // the round queues are sized once, for round_paths paths, and charged to
     // the budget before any scene or job
     const int rc = ensure_round(s, o.round_paths);
@@ -569,7 +584,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
     uint32_t *ctr = s->counters.as<uint32_t>();
     SegLaunch L;
     SHIPX(s, hipMemsetAsync(ctr + kSegBase, 0, sizeof(uint32_t) * kCtr * J, s->stream));
-    uint32_t grid = make_launch(L, npaths.data(), J);
+    uint32_t grid = make_persistent_launch(L, npaths.data(), J, s->resident_shade);
     hipLaunchKernelGGL(k_seg_raygen, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[0], Lpath);
     SHIPX(s, hipGetLastError());
     std::vector<uint32_t> cnt(npaths), hc(4 * J), nsr(J), nrec(J);
@@ -591,14 +606,15 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
                                buf[cur].d, static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>(),
                                static_cast<float4 *>(nullptr), ctr);
         } else {
-            grid = make_launch(L, cnt.data(), J);
+            grid = make_dense_launch(L, cnt.data(), J);
             hipLaunchKernelGGL((k_seg_trace<false>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, buf[cur].o, buf[cur].d,
                                static_cast<const float4 *>(nullptr), s->hitA.as<uint4>(), s->hitB.as<float>());
         }
         SHIPX(s, hipGetLastError());
-        grid = make_launch(L, cnt.data(), J);
+        SegLaunch Ls;
+        grid = make_persistent_launch(Ls, cnt.data(), J, s->resident_shade);
         SHIPX(s, hipMemsetAsync(ctr, 0, 16 * kCtr * J, s->stream));
-        hipLaunchKernelGGL(k_seg_shade, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, b, buf[cur],
+        hipLaunchKernelGGL(k_seg_shade, dim3(grid), dim3(kBlock), 0, s->stream, Ls, jobs, b, buf[cur],
                            s->hitA.as<const uint4>(), s->hitB.as<const float>(), buf[cur ^ 1], sr[0], nee, Lpath, ctr);
         SHIPX(s, hipGetLastError());
         if ((rc = read_job_counters(s, J, hc.data()))) return rc;
@@ -627,7 +643,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
                 for (uint32_t j = 0; j < J; j++) tsr += nsr[j];
                 if (tsr == 0) break;
                 s->stats.rays_shadow += tsr;
-                grid = make_launch(L, nsr.data(), J);
+                grid = make_dense_launch(L, nsr.data(), J);
                 hipLaunchKernelGGL((k_seg_trace<true>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q].o, sr[q].d,
                                    sr[q].lp, s->hitA.as<uint4>(), s->hitB.as<float>());
                 SHIPX(s, hipGetLastError());
@@ -641,7 +657,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
             }
         }
         if (any_rec) {
-            grid = make_launch(L, nrec.data(), J);
+            grid = make_persistent_launch(L, nrec.data(), J, s->resident_shade);
             hipLaunchKernelGGL(k_seg_nee_finalize, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, nee, Lpath);
             SHIPX(s, hipGetLastError());
         }
@@ -649,7 +665,7 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
     }
     std::vector<uint32_t> pix(J);
     for (uint32_t j = 0; j < J; j++) pix[j] = rj[j]->P;
-    grid = make_launch(L, pix.data(), J);
+    grid = make_persistent_launch(L, pix.data(), J, s->resident_shade);
     hipLaunchKernelGGL(k_seg_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, Lpath);
     SHIPX(s, hipGetLastError());
     {
