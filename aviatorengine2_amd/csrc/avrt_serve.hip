@@ -183,17 +183,55 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_seg_shadow_resolve(SegLaunch L, const JobDev *__restrict__ jobs,
-                                                               ShadowRays in, const uint4 *__restrict__ hitA,
-                                                               ShadowRays out, float4 *contrib, uint32_t *counters)
+// Shadow rays of schedule 0, one ray per lane, with the glass steps taken in
+// the same lane: the closest hit before the light point; an opaque one zeroes
+// the ray's contribution slot, glass continues the ray from just past the hit
+// (at most 4 segments), a light or nothing keeps the contribution. The same
+// decisions as the closest-hit launches plus k_seg_shadow_resolve, without
+// the host reading counts between the segments. Continuation segments are
+// counted per job for the round.
+__global__ __launch_bounds__(kBlock) void k_seg_shadow_walk(SegLaunch L, const JobDev *__restrict__ jobs, ShadowRays sr,
+                                                            float4 *contrib, uint32_t *ctr)
 {
+    __shared__ uint32_t lds_stack[kStackLds * kBlock];
     const uint32_t j = job_of(L, blockIdx.x);
-    const uint32_t stride = (L.blk[j + 1] - L.blk[j]) * kBlock, n = L.count[j];
-    const uint32_t s = 4u * jobs[j].path_base;
+    const uint32_t local = (blockIdx.x - L.blk[j]) * kBlock + threadIdx.x;
     const SceneDev sc = jobs[j].sc;
-    for (uint32_t k0 = (blockIdx.x - L.blk[j]) * kBlock; k0 < n; k0 += stride)
-        shadow_resolve_ray(sc, k0 + threadIdx.x, n, at(in, s), hitA + s, at(out, s), contrib + s,
-                           counters + 4u * kCtr * j + 3u * kCtr);
+    const uint32_t base = 4u * jobs[j].path_base;
+    uint32_t nseg = 0;
+    if (local < L.count[j]) {
+        const uint32_t idx = base + local;
+        const float4 O = sr.o[idx], D = sr.d[idx], Lp = sr.lp[idx];
+        const F3 dir = f3(D.x, D.y, D.z), lp = f3(Lp.x, Lp.y, Lp.z);
+        const uint32_t slot = __float_as_uint(D.w);
+        uint32_t step = __float_as_uint(Lp.w);
+        uint32_t spill_mem[kStackSpill];
+        Traversal<false, kStackLds, kBlock, kStackSpill> tr;
+        F3 so = f3(O.x, O.y, O.z);
+        bool more = tr.init(sc, so, dir, O.w, 0.001f, length(lp - so) - 0.001f, lds_stack, spill_mem);
+        while (more) {
+            while (tr.step(sc)) {
+            }
+            const float t = tr.hit.t;
+            const float bound = length(lp - so) - 0.001f;
+            more = false;
+            if (tr.hit.obj != ~0u && !(t >= bound)) {
+                const uint32_t ct = sc.objects[tr.hit.obj].colorType;
+                if (ct != AVRT_DIFFUSE_LIGHT) {
+                    if (ct != AVRT_DIELECTRIC) {
+                        contrib[base + slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    } else if (step + 1u < 4u) {
+                        so = (so + dir * t) + (dir * 0.001f);
+                        step = step + 1u;
+                        nseg++;
+                        more = tr.init(sc, so, dir, O.w, 0.001f, length(lp - so) - 0.001f, lds_stack, spill_mem);
+                    }
+                }
+            }
+        }
+    }
+    const uint32_t total = wave_sum(nseg);
+    if (__lane_id() == 0 && total != 0u) atomicAdd(ctr + kSegBase + j * kCtr, total);
 }
 
 __global__ __launch_bounds__(kBlock) void k_seg_nee_finalize(SegLaunch L, const JobDev *__restrict__ jobs, NeeBufs nee,
@@ -635,26 +673,12 @@ int avrt_server_step(avrt_server *s, uint32_t *completed)
             hipLaunchKernelGGL((k_seg_trace_ww<true>), dim3(grid), dim3(kBlock), 0, s->stream, Lt, jobs, sr[0].o,
                                sr[0].d, sr[0].lp, s->hitA.as<uint4>(), s->hitB.as<float>(), nee.contrib, ctr);
             SHIPX(s, hipGetLastError());
-        } else if (any_rec) {
-            // closest hits, then the glass steps on the host's count (up to 4 segments)
-            int q = 0;
-            for (int step = 0; step < 4; step++) {
-                tsr = 0;
-                for (uint32_t j = 0; j < J; j++) tsr += nsr[j];
-                if (tsr == 0) break;
-                s->stats.rays_shadow += tsr;
-                grid = make_dense_launch(L, nsr.data(), J);
-                hipLaunchKernelGGL((k_seg_trace<true>), dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q].o, sr[q].d,
-                                   sr[q].lp, s->hitA.as<uint4>(), s->hitB.as<float>());
-                SHIPX(s, hipGetLastError());
-                SHIPX(s, hipMemsetAsync(ctr, 0, 16 * kCtr * J, s->stream));
-                hipLaunchKernelGGL(k_seg_shadow_resolve, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[q],
-                                   s->hitA.as<const uint4>(), sr[q ^ 1], nee.contrib, ctr);
-                SHIPX(s, hipGetLastError());
-                if ((rc = read_job_counters(s, J, hc.data()))) return rc;
-                for (uint32_t j = 0; j < J; j++) nsr[j] = hc[4 * j + 3];
-                q ^= 1;
-            }
+        } else if (any_rec && tsr) {
+            // one launch: the glass steps are taken in the lanes
+            s->stats.rays_shadow += tsr;
+            grid = make_dense_launch(L, nsr.data(), J);
+            hipLaunchKernelGGL(k_seg_shadow_walk, dim3(grid), dim3(kBlock), 0, s->stream, L, jobs, sr[0], nee.contrib, ctr);
+            SHIPX(s, hipGetLastError());
         }
         if (any_rec) {
             grid = make_persistent_launch(L, nrec.data(), J, s->resident_shade);

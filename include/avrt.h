@@ -473,8 +473,9 @@ typedef struct AvrtServerOptions {
                                    every job must fit. The round queues (about
                                    660 bytes per path) are allocated at creation
                                    and charged to the budget first.             */
-    uint32_t trace_schedule;    /* 0: one ray per lane, glass steps counted on
-                                   the host (the faster on small scenes);
+    uint32_t trace_schedule;    /* 0: one ray per lane, a shadow ray's glass
+                                   steps taken in its lane (the faster on
+                                   small scenes);
                                    1: persistent pass walk, each workgroup on
                                    one job, shadow rays resolved in the walk;
                                    both give the same images                   */
