@@ -186,10 +186,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_shade(SegLaunch L, const JobDev 
 // Shadow rays of schedule 0, one ray per lane, with the glass steps taken in
 // the same lane: the closest hit before the light point; an opaque one zeroes
 // the ray's contribution slot, glass continues the ray from just past the hit
-// (at most 4 segments), a light or nothing keeps the contribution. The same
-// decisions as the closest-hit launches plus k_seg_shadow_resolve, without
-// the host reading counts between the segments. Continuation segments are
-// counted per job for the round.
+// (at most 4 segments), a light or nothing keeps the contribution: the
+// decisions of shadow_resolve_ray, taken without a host read of the counts
+// between segments. Continuation segments are counted per job for the round.
 __global__ __launch_bounds__(kBlock) void k_seg_shadow_walk(SegLaunch L, const JobDev *__restrict__ jobs, ShadowRays sr,
                                                             float4 *contrib, uint32_t *ctr)
 {
