@@ -112,6 +112,7 @@ struct avrt_ctx {
     std::vector<uint32_t> refit_tree;                // uint32[3] per node: tree base, vertexBase, indexBase
     std::vector<uint32_t> refit_meshes;              // uint32[4] per mesh: vertexBase, indexBase, triangles, tree base
     bool refit_dirty = true;                         // schedule changed since the device copy
+    std::vector<uint32_t> refit_levels;              // start of each depth level in the deepest-first order, then the end
     std::vector<avrt::TriPre> h_tris;
     std::unordered_map<uint32_t, uint32_t> blas_root; // blasBase -> super-root
     std::unordered_map<uint32_t, uint32_t> tri_vbase;  // indexBase -> vertexBase

@@ -1073,21 +1073,25 @@ int refit(avrt_ctx *c)
             (rc = upload_vec(c, c->d_blas2_src, c->h_blas2_src.data(), 4 * c->h_blas2_src.size())) ||
             (rc = ensure(c, c->d_refit_box, sizeof(Box6) * c->h_blas.size())))
             return rc;
+        // the level boundaries of the deepest-first order, kept with it
+        c->refit_levels.clear();
+        size_t i = 0;
+        while (i < nn) {
+            size_t j = i;
+            while (j < nn && c->refit_depth[order[j]] == c->refit_depth[order[i]]) j++;
+            c->refit_levels.push_back(static_cast<uint32_t>(i));
+            i = j;
+        }
+        c->refit_levels.push_back(static_cast<uint32_t>(nn));
         c->refit_dirty = false;
     }
-    std::vector<uint32_t> depth_sorted(c->refit_depth);
-    std::sort(depth_sorted.begin(), depth_sorted.end(), std::greater<uint32_t>());
-    size_t i = 0;
-    while (i < nn) {
-        size_t j = i;
-        while (j < nn && depth_sorted[j] == depth_sorted[i]) j++;
-        const uint32_t cnt = static_cast<uint32_t>(j - i);
+    for (size_t l = 0; l + 1 < c->refit_levels.size(); l++) {
+        const uint32_t i = c->refit_levels[l], cnt = c->refit_levels[l + 1] - i;
         hipLaunchKernelGGL(k_refit_level, dim3(blocks(cnt, 256)), dim3(256), 0, c->stream, cnt,
                            c->d_refit_node.as<uint32_t>() + i, c->d_refit_tree.as<uint32_t>() + 3 * i,
                            c->d_verts.as<AvrtVertex>(), c->d_indices.as<uint32_t>(), c->d_blas.as<AvrtBVHNode>(),
                            c->d_refit_box.as<Box6>());
         HIPX(c, hipGetLastError());
-        i = j;
     }
     const uint32_t n4 = static_cast<uint32_t>(c->h_blas2.size());
     hipLaunchKernelGGL(k_refit_node4, dim3(blocks(4 * n4, 256)), dim3(256), 0, c->stream, n4,
