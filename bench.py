@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--trace-schedule", type=int, default=None,
                     help="0 persistent node/leaf passes (default), 1 one ray per lane, 2 one-step persistent walk")
     ap.add_argument("--served-jobs", type=int, default=160, help="jobs of the served stream (0 skips the leg)")
+    ap.add_argument("--latency-jobs", type=int, default=1000,
+                    help="jobs of the Poisson run that the p50/p99 latency is taken over")
     ap.add_argument("--served-load", type=float, default=0.6,
                     help="Poisson arrival rate of the latency run, as a fraction of the measured served capacity")
     ap.add_argument("--hybrid-frames", type=int, default=16, help="frames of the hybrid orbit (0 skips the leg)")
@@ -257,7 +259,7 @@ def run_served(args, plain_mrays):
         samples = (st["paths"] - s0["paths"]) / wall
         log(f"served throughput: {mrays:.1f} Mrays/s, {cap_jobs:.1f} jobs/s over {wall:.1f}s, {st['rounds'] - s0['rounds']} rounds")
         rate = args.served_load * cap_jobs
-        ljobs = serve.poisson_jobs(args.served_jobs, rate_hz=rate, seed=0x10B6)
+        ljobs = serve.poisson_jobs(args.latency_jobs, rate_hz=rate, seed=0x10B6)
         llat, lwall, lst, _ = serve.run_stream(srv, ids, scene_list, ljobs, progress=log)
         p50, p99 = float(np.percentile(llat, 50)) * 1e3, float(np.percentile(llat, 99)) * 1e3
         log(f"served latency at {rate:.1f} jobs/s: p50 {p50:.1f} ms, p99 {p99:.1f} ms")
