@@ -184,16 +184,20 @@ __global__ void k_bounds(uint32_t n, const AvrtObject *objects, const int32_t *m
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup-wide exclusive scan of n values (chunks of kBT), out[n] = total.
+// Workgroup-wide exclusive scan of n values, out[n] = total. Each thread
+// takes two consecutive values, so a chunk covers 2 * kBT values: the scans
+// of one level of a tree over up to 2048 objects need one pass of barriers.
 // ---------------------------------------------------------------------------
 __device__ void wg_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *sh_wave, uint32_t *sh_carry)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     if (tid == 0) *sh_carry = 0u;
     __syncthreads();
-    for (uint32_t base = 0; base < n; base += kBT) {
-        const uint32_t i = base + tid;
-        const uint32_t v = i < n ? in[i] : 0u;
+    for (uint32_t base = 0; base < n; base += 2u * kBT) {
+        const uint32_t i0 = base + 2u * tid, i1 = i0 + 1u;
+        const uint32_t v0 = i0 < n ? in[i0] : 0u;
+        const uint32_t v1 = i1 < n ? in[i1] : 0u;
+        const uint32_t v = v0 + v1;
         uint32_t x = v;
         for (uint32_t off = 1; off < 64; off <<= 1) {
             const uint32_t y = __shfl_up(x, off);
@@ -212,7 +216,8 @@ __device__ void wg_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t 
         }
         __syncthreads();
         const uint32_t excl = x - v + sh_wave[wave] + *sh_carry;
-        if (i < n) out[i] = excl;
+        if (i0 < n) out[i0] = excl;
+        if (i1 < n) out[i1] = excl + v0;
         __syncthreads();
         if (tid == kBT - 1) *sh_carry = excl + v;
         __syncthreads();
