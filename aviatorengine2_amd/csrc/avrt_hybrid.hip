@@ -48,6 +48,10 @@ using namespace avrt::dev;
 namespace {
 
 constexpr uint32_t kSrcSky = 0, kSrcRaster = 1, kSrcAnalytic = 2;
+// Queue counters: [0] continuation rays, [kCtr] shadow requests, [2 kCtr] emitted
+// shadow rays, each on its own 128-byte line so that the atomics of one stage
+// do not serialise against another's.
+constexpr size_t kHyCounterBytes = 4 * kCtr * sizeof(uint32_t);
 
 __device__ __forceinline__ unsigned short h_bits(float x) { return __half_as_ushort(__float2half_rn(x)); }
 __device__ __forceinline__ float h_val(unsigned short b) { return __half2float(__ushort_as_half(b)); }
@@ -327,7 +331,7 @@ __device__ void def_scatter(const SceneDev &sc, const FrameParams &fp, const Avr
 __device__ __forceinline__ void emit_nee(const NeeOut &nee, const Surf &s, float time, F3 tp, uint32_t pid,
                                          uint32_t nl, Queries q, NeeBufs rec, uint32_t *counters)
 {
-    const uint32_t r = wave_append(nee.want, &counters[1]);
+    const uint32_t r = wave_append(nee.want, &counters[kCtr]);
     if (nee.want) {
         rec.tp[r] = make_float4(tp.x, tp.y, tp.z, __uint_as_float(pid));
         for (uint32_t i = 0; i < kWaveLights; i++)
@@ -336,7 +340,7 @@ __device__ __forceinline__ void emit_nee(const NeeOut &nee, const Surf &s, float
     const F3 so = s.pos + (s.normal * 0.001f);
     for (uint32_t i = 0; i < kWaveLights; i++) {
         const bool emit = nee.want && nee.lray[i];
-        const uint32_t k = wave_append(emit, &counters[2]);
+        const uint32_t k = wave_append(emit, &counters[2 * kCtr]);
         if (emit) {
             q.o[k] = make_float4(so.x, so.y, so.z, time);
             q.d[k] = make_float4(nee.ldir[i].x, nee.ldir[i].y, nee.ldir[i].z, __uint_as_float(r * nl + i));
@@ -853,7 +857,7 @@ int ensure_hybrid(avrt_ctx *c, uint32_t W, uint32_t H, size_t cells, uint32_t nl
         for (DevBuf *b : f4)
             if ((rc = ensure(c, *b, cells * 16))) return rc;
         if ((rc = ensure(c, h.pr_hit, cells * 16)) || (rc = ensure(c, h.pr_far, cells * 4)) ||
-            (rc = ensure(c, h.rng, cells * 4)) || (rc = ensure(c, h.counters, 64)))
+            (rc = ensure(c, h.rng, cells * 4)) || (rc = ensure(c, h.counters, kHyCounterBytes)))
             return rc;
         h.cap_cells = cells;
     }
@@ -870,8 +874,10 @@ int ensure_hybrid(avrt_ctx *c, uint32_t W, uint32_t H, size_t cells, uint32_t nl
 
 int read_counters4(avrt_ctx *c, uint32_t *out)
 {
-    HIPX(c, hipMemcpyAsync(out, c->hy.counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    uint32_t raw[4 * kCtr];
+    HIPX(c, hipMemcpyAsync(raw, c->hy.counters.p, kHyCounterBytes, hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 4; i++) out[i] = raw[i * kCtr];
     return AVRT_OK;
 }
 
@@ -978,7 +984,7 @@ int avrt_render_hybrid(avrt_ctx *c, const AvrtConfig *cfg)
     NeeBufs rec{h.nee_tp.as<float4>(), h.nee_contrib.as<float4>()};
     uint32_t *ctr = h.counters.as<uint32_t>();
     for (uint32_t sample = 0; sample < hp.spf; sample++) {
-        HIPX(c, hipMemsetAsync(ctr, 0, 16, c->stream));
+        HIPX(c, hipMemsetAsync(ctr, 0, kHyCounterBytes, c->stream));
         PathBufs p0{h.po[0].as<float4>(), h.pd[0].as<float4>(), h.pt[0].as<float4>()};
         if (wave_nee)
             hipLaunchKernelGGL((k_def_shade0<true>), dim3(blocks_of(cells)), dim3(kBlock), 0, c->stream, sc, fp, hp, g,
@@ -1000,7 +1006,7 @@ int avrt_render_hybrid(avrt_ctx *c, const AvrtConfig *cfg)
                                npaths, in.o, in.d, static_cast<const float4 *>(nullptr), h.hitA.as<uint4>(),
                                h.hitB.as<float>());
             HIPX(c, hipGetLastError());
-            HIPX(c, hipMemsetAsync(ctr, 0, 16, c->stream));
+            HIPX(c, hipMemsetAsync(ctr, 0, kHyCounterBytes, c->stream));
             if (wave_nee)
                 hipLaunchKernelGGL((k_def_shade<true>), dim3(blocks_of(npaths)), dim3(kBlock), 0, c->stream, sc, fp,
                                    npaths, bounce, in, h.hitA.as<const uint4>(), h.hitB.as<const float>(), cs, out, q,
